@@ -1,0 +1,163 @@
+/*
+ * wsbb_amd.h -- C ABI of the MI355X (gfx950) flowshop bounding engine.
+ *
+ * Plain C types only: host pointers are caller-owned; pointers named d_* are
+ * device (HBM) pointers obtained from wsbb_malloc.  Every call returns
+ * WSBB_OK (0) or a negative error code and never exits the process;
+ * wsbb_last_error() returns the calling thread's last message.  One context
+ * owns one instance, one HIP stream and its device memory; a context is not
+ * re-entrant (use one per host thread), like the reference's bound object
+ * (src/bound_flowshop.cpp:921, shared `courant`).
+ *
+ * Node encoding (reference raw_bb_problem fields, bb_problem_calculator.cpp
+ * 101-144): a permutation of the n jobs plus limite1 (index of the last job
+ * fixed at the front, -1 if none) and limite2 (index of the first job fixed
+ * at the back, n if none).  Batches are position-major: permT[i*N + k] is
+ * position i of node k (uint8 job ids).
+ */
+#ifndef WSBB_AMD_H
+#define WSBB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WSBB_OK 0
+#define WSBB_EINVAL (-1)   /* bad argument */
+#define WSBB_EHIP (-2)     /* HIP runtime error */
+#define WSBB_ENOMEM (-3)   /* allocation failed */
+#define WSBB_EOVERFLOW (-4) /* device node pool full */
+#define WSBB_EBUDGET (-5)  /* node budget exhausted before proof */
+#define WSBB_ENODEV (-6)   /* no usable GPU */
+
+#define WSBB_LB2 0     /* Johnson two-machine bound over all machine pairs */
+#define WSBB_LB1 1     /* one-machine bound (no counterpart in the reference) */
+#define WSBB_LEARNED 2 /* LB2 over learned pair subsets (search only) */
+
+#define WSBB_NSTATS 12
+/* stats[] of wsbb_solve:
+ *  0 decomposed parents   1 children bounded     2 leaves reached
+ *  3 pruned at pop        4 pruned at insertion  5 incumbent improvements
+ *  6 max pool size        7 rounds               8 subset-bounded children
+ *  9 full-LB2 fallbacks  10 pairs evaluated
+ * 11 algorithmic bound work of the bounded children, in int32 ops:
+ *    per child P(n + 4u + 4) + 2m d1 + m(m+1) d2 (u free jobs, d1/d2
+ *    prefix/suffix lengths) */
+
+int wsbb_version(void);
+const char *wsbb_last_error(void);
+int wsbb_device_count(int *count);
+
+/* ---- instances (host only, no GPU) ---------------------------------------
+ * Taillard generator, src/instance_flowshop.cpp:55-96 (unif 74-82,
+ * generate_instance 84-89).  p is machine-major [m][n]; pass p = NULL to
+ * query n and m only. */
+int wsbb_taillard(int id, int *n, int *m, int32_t *p);
+int wsbb_generate(int n, int m, long seed, int32_t *p);
+/* job-correlated instance: p[k][j] = a_j + U[0,20], a_j ~ U[1,80] */
+int wsbb_generate_correlated(int n, int m, uint64_t seed, int32_t *p);
+/* synthetic node batch: node k draws from splitmix64 seeded seed0 + k:
+ * d1 ~ U{0..n-1}, d2 ~ U{0..n-1-d1}, Fisher-Yates permutation;
+ * l1 = d1 - 1, l2 = n - d2. */
+int wsbb_random_nodes(int n, int N, uint64_t seed0, uint8_t *permT,
+                      int32_t *l1, int32_t *l2);
+/* Instance precompute on the host, replacing bound_flowshop::initialiserVar
+ * (src/bound_flowshop.cpp:556-576): pairs (remplirMachine 500-508), lags
+ * (remplirLag 486-498), Johnson orders (Johnson 627-643 with the quicksort
+ * tie order of 597-625), R0/Q0 (remplirTempsArriverDepart 510-538).
+ * Sizes: pair_a/pair_b [P], lag/johnson [P*n], R0/Q0 [m], P = m(m-1)/2.
+ * Any output pointer may be NULL. */
+int wsbb_precompute(const int32_t *p, int n, int m, int32_t *pair_a,
+                    int32_t *pair_b, int32_t *lag, int32_t *johnson,
+                    int32_t *R0, int32_t *Q0);
+
+/* ---- context ---------------------------------------------------------------
+ * Replaces bound_flowshop::set_instance -> init (src/bound_flowshop.cpp
+ * 911-927, 953-956) together with gpu::init / allocate_on_device /
+ * copy_to_device (src/gpu.cu:126-162, 620-642): precomputes the instance
+ * tables on the host and uploads them to `device`.  Limits: 2 <= n <= 128,
+ * 2 <= m <= 24, processing times in [0, 65535]. */
+typedef struct wsbb_ctx wsbb_ctx;
+int wsbb_create(const int32_t *p, int n, int m, int device, wsbb_ctx **out);
+int wsbb_destroy(wsbb_ctx *ctx);
+int wsbb_info(const wsbb_ctx *ctx, int *n, int *m, int *pairs, int *num_cus);
+
+/* ---- batch bounding ----------------------------------------------------------
+ * Per node equals bound_flowshop::bornes_calculer(perm, limite1, limite2,
+ * couts, best) -> couts[0] (src/bound_flowshop.cpp:929-951): the makespan
+ * for a leaf (limite2 - limite1 == 1), else LB2 (kind WSBB_LB2; with
+ * best != -1 the first running maximum above best, 154-161) or LB1.
+ * Replaces the H2D copy + Evaluate_ON_GPU + D2H of gpu::decompose_on_cpu
+ * (src/gpu.cu:211-234; kernel src/bound_flowshop_gpu.cu:619-674).
+ * Host-buffer form: copies in, bounds, copies out, synchronises. */
+int wsbb_bound_batch(wsbb_ctx *ctx, const uint8_t *permT, const int32_t *l1,
+                     const int32_t *l2, int N, int kind, int best,
+                     int32_t *out);
+/* Host-buffer form with the optional outputs used by tests and the pair
+ * selector: argmax [N] (first pair reaching LB2; the index idxMoinsBonCmax
+ * of borneInfMakespan 150-153), pairv [P*N] (every pair value, [s][k]),
+ * sel [ceil(P/32)*N] (restricts LB2 to a per-node pair subset), feat
+ * [5m*N] (selector features).  NULL disables each. */
+int wsbb_bound_batch_ex(wsbb_ctx *ctx, const uint8_t *permT,
+                        const int32_t *l1, const int32_t *l2, int N, int kind,
+                        int best, int32_t *out, int32_t *argmax,
+                        int32_t *pairv, const uint32_t *sel, float *feat);
+/* Device-resident form: all pointers in HBM, asynchronous on the context's
+ * stream (call wsbb_synchronize before reading d_out). */
+int wsbb_bound_device(wsbb_ctx *ctx, const uint8_t *d_permT,
+                      const int32_t *d_l1, const int32_t *d_l2, int N,
+                      int kind, int32_t *d_out);
+
+/* ---- device memory, stream and timing -------------------------------------- */
+int wsbb_malloc(wsbb_ctx *ctx, size_t bytes, void **d_ptr);
+int wsbb_free(wsbb_ctx *ctx, void *d_ptr);
+int wsbb_h2d(wsbb_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int wsbb_d2h(wsbb_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int wsbb_synchronize(wsbb_ctx *ctx);
+/* HIP events recorded on the context's stream around a timed region */
+int wsbb_timer_start(wsbb_ctx *ctx);
+int wsbb_timer_stop(wsbb_ctx *ctx, float *ms);
+
+/* ---- launch shape and autotuning --------------------------------------------
+ * groups: waves that split the pairs of one 64-node tile (1, 2 or 4);
+ * grid_blocks: cap on 256-thread blocks (0 = automatic).  wsbb_autotune
+ * times every shape on a synthetic batch of N nodes and keeps the fastest
+ * for batches of that size class; shapes/times are written if non-NULL. */
+int wsbb_set_launch(wsbb_ctx *ctx, int groups, int grid_blocks);
+int wsbb_autotune(wsbb_ctx *ctx, int N, int reps, int32_t *shapes,
+                  float *ms, int *count);
+
+/* ---- device-resident branch-and-bound -------------------------------------------
+ * Replaces explorer::_explore_cpu_gpu (src/explorer.cu:154-305) with
+ * gpu::decompose_on_cpu (src/gpu.cu:180-255) and gpu::prune (257-408): the
+ * node pool, parent selection, child generation (decompose_beginend,
+ * src/decompose_beginend.cpp:26-138), bounding, branching-side choice
+ * (choise 39-50), ranking (problems::ranks, src/problems.cpp:46-62 with
+ * select_worst), incumbent update (solutions::insert, src/solutions.cpp
+ * 45-65) and pruning all stay on the device.  Parents are popped depth-first
+ * into up to max_batches sub-batches whose reserved child counts
+ * 2*(n-depth) stay below pool_size-1 (problems::push_back1, problems.cpp
+ * 64-78).  ub0: initial incumbent (INT32_MAX = none).  budget: maximum
+ * decomposed parents (<= 0: none).  kind: WSBB_LB2, WSBB_LB1 or
+ * WSBB_LEARNED (needs wsbb_selector_load).  Outputs: best makespan,
+ * best_perm [n] (may be NULL), stats [WSBB_NSTATS] (may be NULL).
+ * Returns WSBB_EBUDGET (with the incumbent so far) when the budget ran out. */
+int wsbb_solve(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
+               int32_t ub0, int64_t budget, int32_t *best,
+               int32_t *best_perm, int64_t *stats);
+/* Optional HIP-event timing of the bound-kernel launches of the next
+ * wsbb_solve calls (enable = 1/0; -1 only reads).  Reports the summed kernel
+ * time and the number of launches of the last solve.  No counterpart in the
+ * reference (its statistics time whole GPU phases, explorer.cu:219-249). */
+int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
+                        int64_t *bound_launches);
+/* Device node-pool capacity in nodes (default 2^24) for later solves. */
+int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WSBB_AMD_H */

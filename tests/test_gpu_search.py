@@ -1,0 +1,84 @@
+"""Device-resident branch-and-bound against the oracle's search loops.
+
+The device engine follows the reference's batched GPU path, so besides the
+proven optimum every counter (parents decomposed, children bounded, leaves,
+prunes, incumbent updates, pool peak, rounds) must equal the oracle's
+batched explorer on the same instance and pool shape."""
+import numpy as np
+import pytest
+
+import oracle
+import wsbb_amd
+
+pytestmark = pytest.mark.gpu
+
+STATS = ("decomposed", "bounded", "leaves", "pruned_pop", "pruned_insert", "improvements",
+         "max_pool", "rounds")
+
+
+def cmax_np(p, perm):
+    t = np.zeros(p.shape[0], np.int64)
+    for j in perm:
+        t[0] += p[0, j]
+        for k in range(1, p.shape[0]):
+            t[k] = max(t[k], t[k - 1]) + p[k, j]
+    return int(t[-1])
+
+
+def check_solve(p, kind, pool, nb, ref_best=None):
+    o = oracle.Oracle(p)
+    okind = oracle.LB2 if kind == wsbb_amd.LB2 else oracle.LB1
+    best, st, _ = o.solve_batched(kind=okind, pool_size=pool, max_batches=nb)
+    with wsbb_amd.Engine(p) as eng:
+        res = eng.solve(kind=kind, pool_size=pool, max_batches=nb)
+    assert res["proven"]
+    assert res["best"] == best
+    if ref_best is not None:
+        assert best == ref_best
+    assert cmax_np(p, res["perm"]) == res["best"]
+    assert sorted(res["perm"].tolist()) == list(range(p.shape[1]))
+    for k in STATS:
+        assert res["stats"][k] == st[k], (k, res["stats"][k], st[k])
+    return res
+
+
+@pytest.mark.parametrize("tid", range(1, 11))
+def test_20x5_lb2_pool4096(tid):
+    p = oracle.taillard(tid)[3]
+    ref, _, _ = oracle.Oracle(p).solve_dfs()
+    check_solve(p, wsbb_amd.LB2, 4096, 1, ref)
+
+
+@pytest.mark.parametrize("pool,nb", [(4096, 10), (64, 1), (300, 3), (45, 7)])
+def test_20x5_pool_shapes(pool, nb):
+    for tid in (1, 5):
+        check_solve(oracle.taillard(tid)[3], wsbb_amd.LB2, pool, nb)
+
+
+def test_20x5_lb1():
+    for tid in (1, 3):
+        check_solve(oracle.taillard(tid)[3], wsbb_amd.LB1, 4096, 1)
+
+
+def test_small_random_instances():
+    rng = np.random.default_rng(5)
+    for trial in range(12):
+        n, m = int(rng.integers(3, 12)), int(rng.integers(2, 8))
+        p = rng.integers(1, int(rng.choice([4, 99])) + 1, size=(m, n)).astype(np.int32)
+        check_solve(p, wsbb_amd.LB2, 2 * n + 2 + int(rng.integers(0, 50)),
+                    int(rng.integers(1, 4)))
+
+
+def test_ta011_20x10_config():
+    """20x10, pools of 4096 children, ten sub-batches per round."""
+    check_solve(oracle.taillard(11)[3], wsbb_amd.LB2, 4096, 10, 1582)
+
+
+def test_budget_stops_without_proof():
+    p = oracle.taillard(11)[3]
+    with wsbb_amd.Engine(p) as eng:
+        res = eng.solve(pool_size=4096, max_batches=1, budget=500)
+    assert not res["proven"]
+    assert res["stats"]["decomposed"] <= 500
+    if res["perm"][0] >= 0:
+        assert cmax_np(p, res["perm"]) == res["best"]

@@ -1,0 +1,327 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+What pins what (the reference ships no tests, fixtures or golden vectors):
+  - Taillard generator: the reference's own instance_flowshop.cpp compiled into
+    oracle/_ref/ref_generator (all 120 instances + raw unif streams), and the
+    canonical first row of ta001.
+  - Search loops + LB2: published Taillard optima (ta001-ta006, ta008-ta012).
+    ta007 as the reference generates it differs from the canonical instance: a
+    valid permutation of makespan 1234 < 1239 exists (asserted below).
+  - LB2 per pair: equals the brute-force optimum of the two-machine relaxation
+    with lags over every order of the unscheduled jobs (Johnson/Mitten), so
+    the group/key rule, lags, heads, R0, tails, Q0 are fixed independently of
+    the quicksort tie order; tie order itself is shown not to change values.
+  - LB2 validity: never above the best completion (brute force, n <= 7);
+    exact at the root for m = 2 (Johnson's theorem) and with one free job.
+  - heads/tails: makespan identity Cmax = max_k r[k] + q[k] at every split.
+  - LB1 (build-defined, no reference counterpart): validity by brute force and
+    LB1 <= LB2.
+  - DFS heap order (the reference's priority queue) equals the LIFO stack.
+"""
+import itertools
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+REF_GEN = os.path.join(os.path.dirname(oracle.__file__), "_ref", "ref_generator")
+
+# published optimal makespans of Taillard's 20-job instances
+PUBLISHED = {1: 1278, 2: 1359, 3: 1081, 4: 1293, 5: 1235, 6: 1195, 8: 1206, 9: 1230,
+             10: 1108, 11: 1582, 12: 1659}
+
+
+def cmax_np(p, perm):
+    t = np.zeros(p.shape[0], np.int64)
+    for j in perm:
+        t[0] += p[0, j]
+        for k in range(1, p.shape[0]):
+            t[k] = max(t[k], t[k - 1]) + p[k, j]
+    return int(t[-1])
+
+
+def rand_instance(rng, n, m, hi=99):
+    return rng.integers(1, hi + 1, size=(m, n)).astype(np.int32)
+
+
+def rand_node(rng, n, min_free=1):
+    perm = rng.permutation(n).astype(np.int32)
+    d1 = int(rng.integers(0, n - min_free + 1))
+    d2 = int(rng.integers(0, n - min_free - d1 + 1))
+    return perm, d1 - 1, n - d2
+
+
+# ---------------------------------------------------------------- generator
+@pytest.mark.skipif(not os.path.exists(REF_GEN), reason="oracle/_ref not built")
+def test_generator_matches_reference_build():
+    out = subprocess.run([REF_GEN] + [str(i) for i in range(1, 121)], capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    assert len(out) == 120
+    for line in out:
+        vals = [int(v) for v in line.split()]
+        tid, N, M, seed = vals[:4]
+        n, m, s, p = oracle.taillard(tid)
+        assert (n, m, s) == (N, M, seed)
+        assert vals[4:] == p.reshape(-1).tolist(), tid
+
+
+@pytest.mark.skipif(not os.path.exists(REF_GEN), reason="oracle/_ref not built")
+def test_unif_stream_matches_reference_build():
+    for seed in (1, 12345, 2**31 - 2, 873654221, 987654321):
+        out = subprocess.run([REF_GEN, "unif", str(seed), "2000"], capture_output=True,
+                             text=True, check=True).stdout.split()
+        assert [int(v) for v in out] == oracle.unif_sequence(seed, 2000)
+
+
+def test_ta001_first_row_is_canonical():
+    n, m, seed, p = oracle.taillard(1)
+    assert (n, m, seed) == (20, 5, 873654221)
+    assert p[0].tolist() == [54, 83, 15, 71, 77, 36, 53, 38, 27, 87, 76, 91, 14, 29, 12,
+                             77, 32, 87, 68, 94]
+
+
+# ---------------------------------------------------------------- precompute
+def test_tables_closed_forms():
+    rng = np.random.default_rng(1)
+    for n, m in [(7, 2), (9, 5), (20, 10), (13, 7)]:
+        p = rand_instance(rng, n, m)
+        t = oracle.Oracle(p).tables()
+        pairs = [(a, b) for a in range(m) for b in range(a + 1, m)]
+        assert list(zip(t["ma"], t["mb"])) == pairs
+        csum = np.vstack([np.zeros((1, n), np.int64), np.cumsum(p, axis=0)])
+        for s, (a, b) in enumerate(pairs):
+            assert t["lag"][s].tolist() == (csum[b] - csum[a + 1]).tolist()
+        total = csum[m]
+        assert t["R0"].tolist() == [int(csum[k].min()) for k in range(m)]
+        assert t["Q0"].tolist() == [int((total - csum[k + 1]).min()) for k in range(m)]
+
+
+def test_johnson_orders_follow_the_rule():
+    rng = np.random.default_rng(2)
+    for n, m, hi in [(20, 5, 99), (30, 6, 5), (12, 4, 2)]:
+        p = rand_instance(rng, n, m, hi)
+        t = oracle.Oracle(p).tables()
+        for s in range(len(t["ma"])):
+            a, b = t["ma"][s], t["mb"][s]
+            order = t["johnson"][s]
+            assert sorted(order.tolist()) == list(range(n))
+            g = [1 if p[a, j] < p[b, j] else 2 for j in order]
+            key = [(p[a, j] if gg == 1 else p[b, j]) + t["lag"][s, j] for j, gg in zip(order, g)]
+            assert g == sorted(g)
+            k1 = [k for k, gg in zip(key, g) if gg == 1]
+            k2 = [k for k, gg in zip(key, g) if gg == 2]
+            assert k1 == sorted(k1) and k2 == sorted(k2, reverse=True)
+
+
+# ---------------------------------------------------------------- bounds
+def relaxation_values(p, t, perm, l1, l2, r, q):
+    """min over all orders of the unscheduled jobs of each pair's value."""
+    n = p.shape[1]
+    d1, d2 = l1 + 1, n - l2
+    free = perm[l1 + 1:l2]
+    vals = []
+    for s in range(len(t["ma"])):
+        a, b = t["ma"][s], t["mb"][s]
+        best = None
+        for order in itertools.permutations(free):
+            t0 = r[a] if d1 else t["R0"][a]
+            t1 = r[b] if d1 else t["R0"][b]
+            for j in order:
+                t0 += p[a, j]
+                t1 = max(t1, t0 + t["lag"][s, j]) + p[b, j]
+            v = max(t1 + q[b], t0 + q[a]) if d2 else t1 + t["Q0"][b]
+            best = v if best is None else min(best, v)
+        vals.append(best)
+    return vals
+
+
+def test_pair_values_are_relaxation_optima():
+    rng = np.random.default_rng(3)
+    for trial in range(40):
+        n, m = int(rng.integers(4, 8)), int(rng.integers(2, 6))
+        p = rand_instance(rng, n, m, hi=int(rng.choice([3, 20, 99])))
+        o = oracle.Oracle(p)
+        t = o.tables()
+        perm, l1, l2 = rand_node(rng, n, min_free=2)
+        r, q = o.heads_tails(perm, l1, l2)
+        lb, v = o.pair_values(perm, l1, l2)
+        assert v.tolist() == relaxation_values(p, t, perm, l1, l2, r, q)
+        assert lb == max(0, max(v.tolist()))
+
+
+def test_lb_valid_against_brute_force_completions():
+    rng = np.random.default_rng(4)
+    for trial in range(60):
+        n, m = int(rng.integers(3, 8)), int(rng.integers(2, 6))
+        p = rand_instance(rng, n, m)
+        o = oracle.Oracle(p)
+        perm, l1, l2 = rand_node(rng, n)
+        free = perm[l1 + 1:l2]
+        best = min(cmax_np(p, list(perm[:l1 + 1]) + list(f) + list(perm[l2:]))
+                   for f in itertools.permutations(free))
+        lb2 = o.bound(perm, l1, l2)
+        lb1 = o.lb1(perm, l1, l2)
+        assert lb1 <= lb2 <= best
+
+
+def test_m2_root_equals_johnson_optimum():
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        n = int(rng.integers(2, 8))
+        p = rand_instance(rng, n, 2, hi=int(rng.choice([4, 99])))
+        o = oracle.Oracle(p)
+        opt = min(cmax_np(p, s) for s in itertools.permutations(range(n)))
+        assert o.bound(np.arange(n), -1, n) == opt
+
+
+def test_m2_one_free_job_is_exact():
+    rng = np.random.default_rng(6)
+    for trial in range(50):
+        n = int(rng.integers(3, 12))
+        p = rand_instance(rng, n, 2)
+        o = oracle.Oracle(p)
+        perm = rng.permutation(n).astype(np.int32)
+        l1 = int(rng.integers(-1, n - 1))
+        assert o.bound(perm, l1, l1 + 2) == cmax_np(p, perm)
+
+
+def test_leaf_bound_is_makespan_and_split_identity():
+    rng = np.random.default_rng(7)
+    for trial in range(30):
+        n, m = int(rng.integers(2, 25)), int(rng.integers(2, 12))
+        p = rand_instance(rng, n, m)
+        o = oracle.Oracle(p)
+        perm = rng.permutation(n).astype(np.int32)
+        c = cmax_np(p, perm)
+        assert o.cmax(perm) == c
+        for l1 in range(-1, n):
+            assert o.bound(perm, l1, l1 + 1) == c
+            r, q = o.heads_tails(perm, l1, l1 + 1)
+            assert int((r + q).max()) == c
+        r, q = o.heads_tails(perm, n - 1, n)
+        assert r[-1] == c
+        r, q = o.heads_tails(perm, -1, 0)
+        assert q[0] == c
+
+
+def test_tie_order_does_not_change_pair_values():
+    """Any order obeying Johnson's rule gives the same pair values."""
+    rng = np.random.default_rng(8)
+    for trial in range(40):
+        n, m = int(rng.integers(5, 14)), int(rng.integers(2, 6))
+        p = rand_instance(rng, n, m, hi=3)  # many equal keys
+        o = oracle.Oracle(p)
+        t = o.tables()
+        perm, l1, l2 = rand_node(rng, n, min_free=2)
+        r, q = o.heads_tails(perm, l1, l2)
+        lb, v = o.pair_values(perm, l1, l2)
+        free = set(perm[l1 + 1:l2].tolist())
+        d1, d2 = l1 + 1, n - l2
+        for s in range(len(t["ma"])):
+            a, b = t["ma"][s], t["mb"][s]
+            keys = [(0, p[a, j] + t["lag"][s, j]) if p[a, j] < p[b, j]
+                    else (1, -(p[b, j] + t["lag"][s, j])) for j in range(n)]
+            alt = sorted(range(n), key=lambda j: (keys[j], -j))  # reversed ties
+            t0 = r[a] if d1 else t["R0"][a]
+            t1 = r[b] if d1 else t["R0"][b]
+            for j in alt:
+                if j in free:
+                    t0 += p[a, j]
+                    t1 = max(t1, t0 + t["lag"][s, j]) + p[b, j]
+            vv = max(t1 + q[b], t0 + q[a]) if d2 else t1 + t["Q0"][b]
+            assert vv == v[s]
+
+
+def test_best_early_exit_semantics():
+    rng = np.random.default_rng(9)
+    p = rand_instance(rng, 15, 6)
+    o = oracle.Oracle(p)
+    for trial in range(200):
+        perm, l1, l2 = rand_node(rng, 15, min_free=2)
+        lb, v = o.pair_values(perm, l1, l2)
+        best = int(rng.integers(lb - 200, lb + 20))
+        above = [x for x in v.tolist() if x > best]
+        expect = above[0] if above else lb
+        assert o.bound(perm, l1, l2, best=best) == expect
+        assert o.bound(perm, l1, l2, best=-1) == lb
+
+
+def test_decompose_children_and_ranks():
+    n, m, seed, p = oracle.taillard(1)
+    o = oracle.Oracle(p)
+    rng = np.random.default_rng(10)
+    for trial in range(30):
+        perm = rng.permutation(n).astype(np.int32)
+        d1 = int(rng.integers(0, n - 2))
+        d2 = int(rng.integers(0, n - 2 - d1 + 1))
+        l1, l2 = d1 - 1, n - d2
+        depth = d1 + d2
+        res = o.decompose(perm, l1, l2, depth)
+        u = n - depth
+        assert res["nbounded"] == (2 if depth == n - 2 else 2 * u)
+        cnt = len(res["cost"])
+        for c in range(cnt):
+            cp = res["perm"][c]
+            assert sorted(cp.tolist()) == list(range(n))
+            assert res["cost"][c] == o.bound(cp, res["l1"][c], res["l2"][c])
+        # ranks: ascending (cost, job); rows are stored worst first
+        order = sorted(range(cnt), key=lambda c: (res["cost"][c], res["job"][c]))
+        assert [res["rank"][c] for c in order] == list(range(cnt))
+        assert res["rank"].tolist() == list(range(cnt - 1, -1, -1))
+
+
+# ---------------------------------------------------------------- search
+@pytest.mark.parametrize("tid", [1, 2, 3, 4, 6, 8, 9, 10])
+def test_published_optima_20x5(tid):
+    n, m, seed, p = oracle.taillard(tid)
+    best, st, perm = oracle.Oracle(p).solve_dfs()
+    assert best == PUBLISHED[tid]
+    assert cmax_np(p, perm) == best
+
+
+def test_ta007_reference_instance_optimum():
+    n, m, seed, p = oracle.taillard(7)
+    best, st, perm = oracle.Oracle(p).solve_dfs()
+    assert best == 1234 and cmax_np(p, perm) == 1234 and sorted(perm) == list(range(n))
+
+
+def test_published_optimum_ta011():
+    n, m, seed, p = oracle.taillard(11)
+    best, st, perm = oracle.Oracle(p).solve_dfs()
+    assert best == PUBLISHED[11] and cmax_np(p, perm) == best
+
+
+def test_heap_order_equals_stack_order():
+    for tid in (1, 3, 5):
+        n, m, seed, p = oracle.taillard(tid)
+        o = oracle.Oracle(p)
+        b0, s0, p0 = o.solve_dfs(order=0)
+        b1, s1, p1 = o.solve_dfs(order=1)
+        assert b0 == b1 and s0 == s1 and p0.tolist() == p1.tolist()
+
+
+def test_batched_and_lb1_searches_reach_the_same_optimum():
+    for tid in (1, 2, 5):
+        n, m, seed, p = oracle.taillard(tid)
+        o = oracle.Oracle(p)
+        ref, _, _ = o.solve_dfs()
+        for pool, nb in [(4096, 1), (4096, 10), (64, 1), (300, 3)]:
+            best, st, perm = o.solve_batched(pool_size=pool, max_batches=nb)
+            assert best == ref and cmax_np(p, perm) == best
+        best, st, _ = o.solve_dfs(kind=oracle.LB1)
+        assert best == ref
+
+
+def test_small_instances_against_brute_force_optimum():
+    rng = np.random.default_rng(11)
+    for trial in range(25):
+        n, m = int(rng.integers(3, 8)), int(rng.integers(2, 6))
+        p = rand_instance(rng, n, m, hi=int(rng.choice([5, 99])))
+        opt = min(cmax_np(p, s) for s in itertools.permutations(range(n)))
+        o = oracle.Oracle(p)
+        assert o.solve_dfs()[0] == opt
+        assert o.solve_dfs(kind=oracle.LB1)[0] == opt
+        assert o.solve_batched(pool_size=2 * n + 2, max_batches=2)[0] == opt

@@ -1,0 +1,81 @@
+"""Host-side product checks that need no GPU: the library loads and exports
+every symbol declared in include/wsbb_amd.h, its instance precompute and
+generators match the oracle bit for bit, and GPU entry points fail cleanly
+without a device."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import wsbb_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "wsbb_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(wsbb_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(wsbb_amd.lib, name), name
+
+
+def instances():
+    rng = np.random.default_rng(0)
+    out = [oracle.taillard(t)[3] for t in (1, 11, 21, 51, 81)]
+    out.append(oracle.generate_correlated(50, 10, 1234))
+    out.append(rng.integers(1, 4, size=(6, 30)).astype(np.int32))   # heavy ties
+    out.append(rng.integers(0, 2, size=(4, 17)).astype(np.int32))   # zeros and ties
+    out.append(rng.integers(1, 100, size=(24, 128)).astype(np.int32))
+    return out
+
+
+@pytest.mark.parametrize("idx", range(9))
+def test_precompute_matches_oracle(idx):
+    p = instances()[idx]
+    mine = wsbb_amd.precompute(p)
+    ref = oracle.Oracle(p).tables()
+    for k in ("ma", "mb", "lag", "johnson", "R0", "Q0"):
+        assert np.array_equal(mine[k], ref[k]), k
+
+
+def test_taillard_generator_matches_oracle():
+    for tid in range(1, 121):
+        assert np.array_equal(wsbb_amd.taillard(tid), oracle.taillard(tid)[3]), tid
+
+
+def test_other_generators_match_oracle():
+    for n, m, seed in [(20, 5, 1), (50, 20, 77), (100, 10, 2**40 + 3)]:
+        assert np.array_equal(wsbb_amd.generate_correlated(n, m, seed),
+                              oracle.generate_correlated(n, m, seed))
+    for n, N, s0 in [(20, 1000, 0x5EED), (50, 4096, 0x5EED), (100, 777, 9)]:
+        a = wsbb_amd.random_nodes(n, N, s0)
+        b = oracle.random_nodes(n, N, s0)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        permT, l1, l2 = a
+        assert (np.sort(permT, axis=0) == np.arange(n)[:, None]).all()
+        d1, d2 = l1 + 1, n - l2
+        assert (d1 >= 0).all() and (d2 >= 0).all() and (d1 + d2 <= n - 1).all()
+
+
+def test_precompute_rejects_out_of_range_instances():
+    with pytest.raises(wsbb_amd.WsbbError):
+        wsbb_amd.precompute(np.ones((5, 129), np.int32))
+    with pytest.raises(wsbb_amd.WsbbError):
+        wsbb_amd.precompute(np.ones((25, 20), np.int32))
+    with pytest.raises(wsbb_amd.WsbbError):
+        wsbb_amd.precompute(np.full((5, 20), 70000, np.int32))
+
+
+@pytest.mark.skipif(wsbb_amd.device_count() > 0, reason="a GPU is present")
+def test_create_without_gpu_fails_cleanly():
+    with pytest.raises(wsbb_amd.WsbbError) as e:
+        wsbb_amd.Engine(oracle.taillard(1)[3])
+    assert e.value.code == -6

@@ -1,0 +1,258 @@
+"""wsbb_amd: MI355X-native flowshop branch-and-bound bounding engine.
+
+Python face of the C ABI in include/wsbb_amd.h (ctypes, numpy host buffers).
+The compute path is the HIP library wsbb_amd/libwsbb_amd.so; importing this
+module fails loudly if it has not been built (python wsbb_amd/build.py).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwsbb_amd.so")
+
+LB2, LB1, LEARNED = 0, 1, 2
+NSTATS = 12
+STAT_NAMES = ("decomposed", "bounded", "leaves", "pruned_pop", "pruned_insert",
+              "improvements", "max_pool", "rounds", "subset_bounded", "fallbacks",
+              "pairs_evaluated", "work_ops")
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUDGET",
+          -6: "ENODEV"}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+
+class WsbbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python wsbb_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "wsbb_version": ([], ctypes.c_int),
+        "wsbb_last_error": ([], ctypes.c_char_p),
+        "wsbb_device_count": ([_i32p], ctypes.c_int),
+        "wsbb_taillard": ([ctypes.c_int, _i32p, _i32p, _i32p], ctypes.c_int),
+        "wsbb_generate": ([ctypes.c_int, ctypes.c_int, ctypes.c_long, _i32p], ctypes.c_int),
+        "wsbb_generate_correlated": ([ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _i32p], ctypes.c_int),
+        "wsbb_random_nodes": ([ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _u8p, _i32p, _i32p], ctypes.c_int),
+        "wsbb_precompute": ([_i32p, ctypes.c_int, ctypes.c_int] + [_i32p] * 6, ctypes.c_int),
+        "wsbb_create": ([_i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "wsbb_destroy": ([_vp], ctypes.c_int),
+        "wsbb_info": ([_vp, _i32p, _i32p, _i32p, _i32p], ctypes.c_int),
+        "wsbb_bound_batch": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p], ctypes.c_int),
+        "wsbb_bound_batch_ex": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _i32p, _i32p, _i32p, _u32p, _f32p], ctypes.c_int),
+        "wsbb_bound_device": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+        "wsbb_malloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
+        "wsbb_free": ([_vp, _vp], ctypes.c_int),
+        "wsbb_h2d": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "wsbb_d2h": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "wsbb_synchronize": ([_vp], ctypes.c_int),
+        "wsbb_timer_start": ([_vp], ctypes.c_int),
+        "wsbb_timer_stop": ([_vp, _f32p], ctypes.c_int),
+        "wsbb_set_launch": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_autotune": ([_vp, ctypes.c_int, ctypes.c_int, _i32p, _f32p, _i32p], ctypes.c_int),
+        "wsbb_solve": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int32, ctypes.c_int64,
+                        _i32p, _i32p, _i64p], ctypes.c_int),
+        "wsbb_search_profile": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _i64p],
+                                ctypes.c_int),
+        "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+lib = _load()
+
+
+def _p(a, t=_i32p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise WsbbError(rc, (what + ": " if what else "") + lib.wsbb_last_error().decode())
+    return rc
+
+
+def device_count():
+    c = ctypes.c_int32()
+    lib.wsbb_device_count(ctypes.byref(c))
+    return c.value
+
+
+def taillard(tid):
+    """Taillard instance `tid`: returns p[m, n] int32 (machine-major)."""
+    n, m = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib.wsbb_taillard(tid, ctypes.byref(n), ctypes.byref(m), None), "taillard")
+    p = np.zeros((m.value, n.value), np.int32)
+    _check(lib.wsbb_taillard(tid, None, None, _p(p)), "taillard")
+    return p
+
+
+def generate(n, m, seed):
+    p = np.zeros((m, n), np.int32)
+    _check(lib.wsbb_generate(n, m, seed, _p(p)))
+    return p
+
+
+def generate_correlated(n, m, seed):
+    p = np.zeros((m, n), np.int32)
+    _check(lib.wsbb_generate_correlated(n, m, seed, _p(p)))
+    return p
+
+
+def random_nodes(n, N, seed0=0x5EED):
+    permT = np.zeros((n, N), np.uint8)
+    l1 = np.zeros(N, np.int32)
+    l2 = np.zeros(N, np.int32)
+    _check(lib.wsbb_random_nodes(n, N, seed0, _p(permT, _u8p), _p(l1), _p(l2)))
+    return permT, l1, l2
+
+
+def precompute(p):
+    p = np.ascontiguousarray(p, np.int32)
+    m, n = p.shape
+    P = m * (m - 1) // 2
+    out = dict(ma=np.zeros(P, np.int32), mb=np.zeros(P, np.int32),
+               lag=np.zeros((P, n), np.int32), johnson=np.zeros((P, n), np.int32),
+               R0=np.zeros(m, np.int32), Q0=np.zeros(m, np.int32))
+    _check(lib.wsbb_precompute(_p(p), n, m, *[_p(out[k]) for k in
+                                              ("ma", "mb", "lag", "johnson", "R0", "Q0")]))
+    return out
+
+
+class Engine:
+    """One device context: an instance's tables in HBM plus a HIP stream."""
+
+    def __init__(self, p, device=0):
+        self.p = np.ascontiguousarray(p, np.int32)
+        self.m, self.n = self.p.shape
+        h = _vp()
+        _check(lib.wsbb_create(_p(self.p), self.n, self.m, device, ctypes.byref(h)), "create")
+        self.h = h
+        P, cus = ctypes.c_int32(), ctypes.c_int32()
+        lib.wsbb_info(self.h, None, None, ctypes.byref(P), ctypes.byref(cus))
+        self.P, self.num_cus = P.value, cus.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.wsbb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- batch bounding (host buffers) ----
+    def bound(self, permT, l1, l2, kind=LB2, best=-1, argmax=False, pairv=False, sel=None,
+              feat=False):
+        permT = np.ascontiguousarray(permT, np.uint8)
+        l1 = np.ascontiguousarray(l1, np.int32)
+        l2 = np.ascontiguousarray(l2, np.int32)
+        N = l1.shape[0]
+        assert permT.shape == (self.n, N) and l2.shape == (N,)
+        out = np.zeros(N, np.int32)
+        am = np.zeros(N, np.int32) if argmax else None
+        pv = np.zeros((self.P, N), np.int32) if pairv else None
+        ft = np.zeros((5 * self.m, N), np.float32) if feat else None
+        if sel is not None:
+            sel = np.ascontiguousarray(sel, np.uint32)
+            assert sel.shape == ((self.P + 31) // 32, N)
+        _check(lib.wsbb_bound_batch_ex(self.h, _p(permT, _u8p), _p(l1), _p(l2), N, kind, best,
+                                       _p(out), _p(am), _p(pv), _p(sel, _u32p), _p(ft, _f32p)),
+               "bound")
+        if not (argmax or pairv or feat):
+            return out
+        res = {"bound": out}
+        if argmax:
+            res["argmax"] = am
+        if pairv:
+            res["pairv"] = pv
+        if feat:
+            res["feat"] = ft
+        return res
+
+    # ---- device memory ----
+    def malloc(self, nbytes):
+        d = _vp()
+        _check(lib.wsbb_malloc(self.h, nbytes, ctypes.byref(d)), "malloc")
+        return d.value
+
+    def free(self, d):
+        _check(lib.wsbb_free(self.h, d), "free")
+
+    def h2d(self, d, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib.wsbb_h2d(self.h, d, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def d2h(self, arr, d):
+        _check(lib.wsbb_d2h(self.h, arr.ctypes.data, d, arr.nbytes), "d2h")
+        return arr
+
+    def bound_device(self, d_perm, d_l1, d_l2, N, d_out, kind=LB2):
+        _check(lib.wsbb_bound_device(self.h, d_perm, d_l1, d_l2, N, kind, d_out), "bound_device")
+
+    def synchronize(self):
+        _check(lib.wsbb_synchronize(self.h), "synchronize")
+
+    def timer_start(self):
+        _check(lib.wsbb_timer_start(self.h), "timer_start")
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        _check(lib.wsbb_timer_stop(self.h, ctypes.byref(ms)), "timer_stop")
+        return ms.value
+
+    def set_launch(self, groups=0, grid_blocks=0):
+        _check(lib.wsbb_set_launch(self.h, groups, grid_blocks), "set_launch")
+
+    def autotune(self, N, reps=3):
+        shapes = np.zeros((16, 2), np.int32)
+        ms = np.zeros(16, np.float32)
+        cnt = ctypes.c_int32()
+        _check(lib.wsbb_autotune(self.h, N, reps, _p(shapes), _p(ms, _f32p), ctypes.byref(cnt)),
+               "autotune")
+        k = cnt.value
+        return [(int(g), int(b), float(t)) for (g, b), t in zip(shapes[:k], ms[:k])]
+
+    # ---- search ----
+    def search_profile(self, enable=-1):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        _check(lib.wsbb_search_profile(self.h, enable, ctypes.byref(ms), ctypes.byref(cnt)),
+               "search_profile")
+        return ms.value, cnt.value
+
+    def set_pool_capacity(self, nodes):
+        _check(lib.wsbb_set_pool_capacity(self.h, nodes), "set_pool_capacity")
+
+    def solve(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
+        best = ctypes.c_int32()
+        bp = np.zeros(self.n, np.int32)
+        st = np.zeros(NSTATS, np.int64)
+        rc = lib.wsbb_solve(self.h, kind, pool_size, max_batches, ub0, budget,
+                            ctypes.byref(best), _p(bp), _p(st, _i64p))
+        if rc not in (0, -5):
+            _check(rc, "solve")
+        return dict(best=best.value, perm=bp, stats=dict(zip(STAT_NAMES, st.tolist())),
+                    proven=(rc == 0))

@@ -1,0 +1,263 @@
+// Node bound kernels for the permutation flowshop on gfx950.
+//
+// One lane bounds one node.  All lanes of a wavefront walk the same machine
+// pair and the same Johnson step at the same time, so every per-step constant
+// (job id, p_a, p_b, lag) is wave-uniform and arrives through scalar loads
+// into SGPRs; the per-lane work is one bit test of the node's unscheduled-job
+// mask and four integer ops.  Heads, tails and the unscheduled mask live in
+// VGPRs; the processing-time matrix is staged in LDS for the per-lane gathers
+// of the head/tail recurrences.  `groups` waves may split the pair range of
+// one 64-node tile (pairs-per-wavefront), combining through LDS.
+//
+// Semantics follow bound_flowshop::bornes_calculer (src/bound_flowshop.cpp
+// 929-951): a leaf (limite2 - limite1 == 1) yields its makespan
+// (criteres_calculer 897-909); otherwise LB2 = max(0, max_s v_s) over the
+// pairs in order (borneInfMakespan 121-168), with v_s built from the head
+// start (initCmax 79-90), the Johnson scan over unscheduled jobs
+// (heuristiqueCmax 64-77) and the tail combine (cmaxFin 92-100 / 147-148).
+#include "bound.h"
+
+namespace wsbb {
+
+namespace {
+
+template <int NW>
+__device__ __forceinline__ void set_job_bit(uint32_t (&sm)[NW], int j) {
+#pragma unroll
+  for (int w = 0; w < NW; ++w) sm[w] |= ((j >> 5) == w) ? (1u << (j & 31)) : 0u;
+}
+
+template <int MM, int NW, class Src>
+__global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
+                                                    BoundArgs A) {
+  extern __shared__ int32_t lds[];
+  const int n = T.n, m = T.m, P = T.P;
+  const int mn4 = (m * n + 3) & ~3;
+  int32_t* sp = lds;                                 // [m][n]
+  int4* part = reinterpret_cast<int4*>(lds + mn4);   // [blockDim] partials
+  for (int i = threadIdx.x; i < m * n; i += blockDim.x) sp[i] = T.p[i];
+  __syncthreads();
+
+  const int G = A.groups;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tpb = (blockDim.x >> 6) / G;  // 64-node tiles per block
+  const int tib = wave / G;
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // pair loop, its table loads and the register-array indices scalar
+  const int grp = __builtin_amdgcn_readfirstlane(wave - tib * G);
+  const int s_lo = (A.kind == 0) ? (grp * P) / G : 0;
+  const int s_hi = (A.kind == 0) ? ((grp + 1) * P) / G : 0;
+  const int N = (A.Ndev != nullptr) ? *A.Ndev : A.N;
+  const size_t ld = A.ld > 0 ? static_cast<size_t>(A.ld) : static_cast<size_t>(N);
+  const int ntiles = (N + 63) >> 6;
+  const int nbt = (ntiles + tpb - 1) / tpb;
+
+  uint32_t valid_bits[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const int hi = n - 32 * w;
+    valid_bits[w] = hi >= 32 ? 0xffffffffu : (hi > 0 ? ((1u << hi) - 1u) : 0u);
+  }
+
+  for (int bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
+    const int node = (bt * tpb + tib) * 64 + lane;
+    int l1 = -1, l2 = n, stride = 0, sw0 = -1, sw1 = -1;
+    const uint8_t* col = nullptr;
+    const bool valid = src.load(node, l1, l2, col, stride, sw0, sw1);
+    if (!valid) {
+      l1 = -1;
+      l2 = n;
+    }
+    const int d1 = l1 + 1, d2 = n - l2;
+
+    // heads of the prefix (set_tempsMachines_retardDebut 841-856) and tails
+    // of the suffix.  The reference runs one forward pass per start machine
+    // (set_tempsMachinesFin_tempsJobFin 876-895); the reversed recurrence
+    // below yields the same longest-path values for all machines at once.
+    // q[k] = tail of machine k; q[k] for k >= m stays 0, which is the
+    // boundary of the reversed recurrence
+    int r[MM], q[MM];
+    uint32_t sm[NW];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sm[w] = 0;
+
+    for (int i = 0; i < d1; ++i) {
+      const int j = perm_at(col, stride, sw0, sw1, i);
+      set_job_bit<NW>(sm, j);
+      r[0] += sp[j];
+#pragma unroll
+      for (int k = 1; k < MM; ++k)
+        if (k < m) r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
+    }
+    for (int i = n - 1; i >= l2; --i) {
+      const int j = perm_at(col, stride, sw0, sw1, i);
+      set_job_bit<NW>(sm, j);
+#pragma unroll
+      for (int k = MM - 1; k >= 0; --k)
+        if (k < m) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
+    }
+    const bool leaf = (l2 - l1 == 1);
+    int leafv = 0;
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) leafv = max(leafv, r[k] + q[k]);
+
+    // one-machine bound and selector features need the remaining work of
+    // every machine: head (or R0) + sum of unscheduled p + tail (or Q0)
+    int lb1 = 0;
+    if (A.kind == 1 || A.feat != nullptr) {
+      int rem[MM];
+#pragma unroll
+      for (int k = 0; k < MM; ++k) rem[k] = 0;
+      const int rem_hi = valid ? l2 : d1;  // padding lanes read no permutation
+      for (int i = d1; i < rem_hi; ++i) {
+        const int j = perm_at(col, stride, sw0, sw1, i);
+#pragma unroll
+        for (int k = 0; k < MM; ++k)
+          if (k < m) rem[k] += sp[k * n + j];
+      }
+#pragma unroll
+      for (int k = 0; k < MM; ++k)
+        if (k < m) {
+          const int head = d1 ? r[k] : ld_const(T.R0, k);
+          const int tail = d2 ? q[k] : ld_const(T.Q0, k);
+          lb1 = max(lb1, head + rem[k] + tail);
+          if (A.feat != nullptr && valid) {
+            const float sc = 1.0f / A.feat_scale;
+            A.feat[(0 * m + k) * ld + node] = head * sc;
+            A.feat[(1 * m + k) * ld + node] = tail * sc;
+            A.feat[(2 * m + k) * ld + node] = rem[k] * sc;
+            A.feat[(3 * m + k) * ld + node] = (head + rem[k]) * sc;
+            A.feat[(4 * m + k) * ld + node] = (rem[k] + tail) * sc;
+          }
+        }
+    }
+
+    uint32_t um[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) um[w] = ~sm[w] & valid_bits[w];
+
+    // pair start values: heads, or R0 without prefix (initCmax 79-90)
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m && d1 == 0) r[k] = ld_const(T.R0, k);
+
+    int lb = 0, fx = -1, am = -1;
+    const bool use_best = A.best != -1;
+    for (int s = s_lo; s < s_hi; ++s) {
+      bool on = true;
+      if (A.sel != nullptr) {
+        on = valid && ((A.sel[static_cast<size_t>(s >> 5) * ld + node] >> (s & 31)) & 1u);
+        if (!__any(on)) continue;
+      }
+      const int2 ab = ld_const2(T.pairs, s);
+      // pair start values: heads, or R0 without prefix (initCmax 79-90)
+      int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
+      int t1 = r[ab.y];
+      const int4* st = T.steps + static_cast<size_t>(s) * T.n4;
+      for (int k = 0; k < T.n4; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int4 q = ld_const4(st, k + u);
+          const uint32_t word = (NW == 1) ? um[0] : um[q.x >> 5];
+          const uint32_t f = (word >> (q.x & 31)) & 1u;
+          // unscheduled job: t0 += p_a; t1 = max(t1, t0 + lag) + p_b.
+          // scheduled job (f = 0): both clocks unchanged, since the biased
+          // t0 stays below t1.
+          T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.y)) + T0);
+          const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.z)) + t1);
+          const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.w)) + T0);
+          t1 = max(x, y);
+        }
+      }
+      // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
+      // counts (borneInfMakespan 146-148)
+      const int qa = q[ab.x], qb = q[ab.y];
+      const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + ld_const(T.Q0, ab.y);
+      if (A.pairv != nullptr && valid) A.pairv[static_cast<size_t>(s) * ld + node] = v;
+      if (on) {
+        if (v > lb) {
+          lb = v;
+          am = s;
+        }
+        if (use_best && fx < 0 && lb > A.best) fx = lb;
+      }
+    }
+
+    if (G > 1) {
+      part[threadIdx.x] = make_int4(lb, fx, am, 0);
+      __syncthreads();
+      if (grp == 0) {
+        int blb = 0, bfx = -1, bam = -1;
+        for (int g = 0; g < G; ++g) {
+          const int4 e = part[(tib * G + g) * 64 + lane];
+          if (bfx < 0 && e.y >= 0) bfx = e.y;
+          if (e.x > blb) {
+            blb = e.x;
+            bam = e.z;
+          }
+        }
+        lb = blb;
+        fx = bfx;
+        am = bam;
+      }
+      __syncthreads();
+    }
+
+    if (grp == 0 && valid) {
+      int res;
+      if (leaf) res = leafv;
+      else if (A.kind == 1) res = lb1;
+      else res = fx >= 0 ? fx : lb;
+      A.out[node] = res;
+      if (A.argmax != nullptr) A.argmax[node] = leaf ? -1 : am;
+    }
+  }
+}
+
+template <int MM, int NW, class Src>
+hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
+                        int grid_cap, hipStream_t stream) {
+  const int block = 256;
+  const int tpb = (block / 64) / A.groups;
+  const int ntiles = (A.N + 63) / 64;
+  int grid = (ntiles + tpb - 1) / tpb;
+  if (grid > grid_cap) grid = grid_cap;
+  if (grid < 1) grid = 1;
+  const size_t shmem = sizeof(int32_t) * (((T.m * T.n + 3) & ~3) + 4 * block);
+  hipLaunchKernelGGL((bound_kernel<MM, NW, Src>), dim3(grid), dim3(block), shmem,
+                     stream, T, src, A);
+  return hipGetLastError();
+}
+
+template <class Src>
+hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
+                      int grid_cap, hipStream_t stream) {
+  const int nw = T.n <= 32 ? 1 : (T.n <= 64 ? 2 : 4);
+  const int mm = T.m <= 8 ? 8 : (T.m <= 16 ? 16 : 24);
+#define WSBB_DISPATCH(MMv, NWv) \
+  if (mm == MMv && nw == NWv) return launch_mm_nw<MMv, NWv, Src>(T, src, A, grid_cap, stream);
+  WSBB_DISPATCH(8, 1) WSBB_DISPATCH(8, 2) WSBB_DISPATCH(8, 4)
+  WSBB_DISPATCH(16, 1) WSBB_DISPATCH(16, 2) WSBB_DISPATCH(16, 4)
+  WSBB_DISPATCH(24, 1) WSBB_DISPATCH(24, 2) WSBB_DISPATCH(24, 4)
+#undef WSBB_DISPATCH
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_bound(const DevTables& T, const BatchSource& src,
+                        const BoundArgs& A, int grid_cap, hipStream_t stream) {
+  if (A.N <= 0) return hipSuccess;
+  return launch_src(T, src, A, grid_cap, stream);
+}
+
+hipError_t launch_bound(const DevTables& T, const ChildSource& src,
+                        const BoundArgs& A, int grid_cap, hipStream_t stream) {
+  if (A.N <= 0) return hipSuccess;
+  return launch_src(T, src, A, grid_cap, stream);
+}
+
+}  // namespace wsbb

@@ -1,0 +1,75 @@
+// Library-private state behind the wsbb_ctx handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "instance.h"
+
+struct wsbb_ctx;
+
+namespace wsbb {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&ptr, need);
+    if (e == hipSuccess) bytes = need;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as(size_t byte_offset = 0) const {
+    return reinterpret_cast<T*>(static_cast<char*>(ptr) + byte_offset);
+  }
+};
+
+struct SearchState;    // search.hip
+struct SelectorState;  // selector.hip
+
+struct Launch {
+  int groups = 1;
+  int grid_cap = 0;  // 0: automatic
+};
+
+}  // namespace wsbb
+
+struct wsbb_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  wsbb::Instance inst;
+  wsbb::DevTables T{};
+  wsbb::DevBuf tables;  // every instance table in one allocation
+  wsbb::DevBuf batch;   // staging for host-buffer batch calls
+  wsbb::Launch launch;
+  int pool_capacity = 1 << 24;  // device node-pool capacity (nodes)
+  bool profile_bound = false;    // time bound launches inside wsbb_solve
+  // autotuned shapes by batch size class (log2 of tiles), -1 = not tuned
+  int tuned_groups[32];
+  wsbb::SearchState* search = nullptr;
+  wsbb::SelectorState* selector = nullptr;
+
+  int grid_cap() const {
+    if (launch.grid_cap > 0) return launch.grid_cap;
+    return num_cus * 8;
+  }
+  int groups_for(int N) const;
+};

@@ -1,0 +1,102 @@
+// Device-side data layouts shared by the bound, search and selector kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "instance.h"
+
+namespace wsbb {
+
+// Loads from tables that no kernel writes while they are read: going through
+// the constant address space lets wave-uniform loads use the scalar unit.
+__device__ __forceinline__ int ld_const(const int* p, size_t i) {
+  return ((const __attribute__((address_space(4))) int*)(p))[i];  // address-space cast
+}
+__device__ __forceinline__ int2 ld_const2(const int2* p, size_t i) {
+  const int* q = reinterpret_cast<const int*>(p) + 2 * i;
+  return make_int2(ld_const(q, 0), ld_const(q, 1));
+}
+__device__ __forceinline__ int4 ld_const4(const int4* p, size_t i) {
+  const int* q = reinterpret_cast<const int*>(p) + 4 * i;
+  return make_int4(ld_const(q, 0), ld_const(q, 1), ld_const(q, 2), ld_const(q, 3));
+}
+
+// Instance tables resident in HBM (built once per instance on the host).
+struct DevTables {
+  int n, m, P;
+  int n4;                // Johnson steps per pair, padded to a multiple of 4
+  const int32_t* p;      // [m][n]
+  const int2* pairs;     // [P] (machine a, machine b)
+  const int4* steps;     // [P][n4] {job, p_a, p_b, kBias + lag + p_b}; pad = 0
+  const int32_t* R0;     // [m]
+  const int32_t* Q0;     // [m]
+  const int32_t* ptot;   // [m]
+};
+
+// What a bound launch computes and where it writes.
+struct BoundArgs {
+  int N;                 // nodes (launch size when Ndev is set)
+  const int* Ndev;       // device-side node count (<= N), or null
+  int ld;                // leading dimension of pairv/sel/feat (0: node count)
+  int kind;              // 0 = LB2, 1 = LB1
+  int best;              // LB2 returns the first pair value above best (-1: off)
+  int groups;            // waves that split the pair range of one 64-node tile
+  int32_t* out;          // [N] bound (leaves: makespan)
+  int32_t* argmax;       // [N] first pair reaching the LB2 value, or null
+  int32_t* pairv;        // [P][N] every pair value, or null
+  const uint32_t* sel;   // [ceil(P/32)][N] pair subset per node, or null
+  float* feat;           // [5m][N] selector features, or null
+  float feat_scale;      // features are divided by this
+};
+
+// Node source: explicit position-major nodes.
+struct BatchSource {
+  const uint8_t* permT;  // [n][N]
+  const int32_t* l1;
+  const int32_t* l2;
+  int N;
+  __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
+                                       int& stride, int& sw0, int& sw1) const {
+    if (k >= N) return false;
+    a = l1[k];
+    b = l2[k];
+    col = permT + k;
+    stride = N;
+    sw0 = sw1 = -1;
+    return true;
+  }
+};
+
+// Node source: children described by (parent, side, position) over a
+// position-major parent array (decompose_beginend child semantics).
+struct ChildSource {
+  const uint8_t* ppermT;  // parents [n][PN]
+  const int32_t* pl1;
+  const int32_t* pl2;
+  const int32_t* desc;    // [N] parent << 9 | side << 8 | j
+  int PN, N;
+  __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
+                                       int& stride, int& sw0, int& sw1) const {
+    if (k >= N) return false;
+    const int d = desc[k];
+    const int par = d >> 9, side = (d >> 8) & 1, j = d & 255;
+    const int pa = pl1[par], pb = pl2[par];
+    // limites_set / permutation_set (bb_problem_calculator.cpp:40-48, 66-74)
+    a = pa + (side == 0);
+    b = pb - (side == 1);
+    if (b - a == 2) a += 1;
+    sw0 = side == 0 ? pa + 1 : pb - 1;
+    sw1 = j;
+    col = ppermT + par;
+    stride = PN;
+    return true;
+  }
+};
+
+__device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,
+                                       int sw1, int i) {
+  const int src = (i == sw0) ? sw1 : ((i == sw1) ? sw0 : i);
+  return col[static_cast<size_t>(src) * stride];
+}
+
+}  // namespace wsbb

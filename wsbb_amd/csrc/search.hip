@@ -1,0 +1,760 @@
+// Device-resident branch-and-bound for the permutation flowshop.
+//
+// The node pool is a LIFO stack in HBM (structure of arrays: position-major
+// uint8 permutations, limits, bound).  Pushing the surviving children of the
+// popped parents last-parent-first and, within a parent, worst-rank-first
+// makes the stack pop nodes in exactly the order of the reference's
+// priority_queue over lexicographic ranks (tree.cpp:37-46): two pool nodes
+// are never ancestor and descendant, so every child of a popped node sorts
+// before every other pool node.
+//
+// One round (all kernels on the context's stream):
+//   select   pop parents into <= max_batches sub-batches, each reserving
+//            2*(n-depth) children below pool_size-1 (explorer.cu:166-208,
+//            problems::push_back1 problems.cpp:64-78); drop parents whose
+//            bound is not below the incumbent (tree::bound, tree.cpp:247-250)
+//   expand   copy parents out of the stack, write child descriptors
+//            (problems_generate3 / leaves_generate3, decompose_beginend.cpp
+//            114-138, in the reference's BEGIN-then-END order)
+//   bound    LB of every child (bound.hip)
+//   rank     per parent: branching side (choise, decompose_beginend.cpp
+//            39-50 with best = 99999), ranks by (bound, job) (problems::ranks,
+//            problems.cpp:46-62 with select_worst), leaf minimum
+//   incumb   incumbent seen by each parent in pop order: the running minimum
+//            of earlier parents' leaves (solutions::insert strictly-better
+//            rule, solutions.cpp:45-65; prune loop gpu.cu:317-394)
+//   keep     children of the chosen side with bound < that incumbent
+//   offsets  prefix sums of kept children, overflow check
+//   push     write survivors onto the stack
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wsbb_amd.h"
+#include "bound.h"
+#include "context.h"
+#include "search.h"
+
+namespace wsbb {
+
+namespace {
+
+enum : int { kRunning = 0, kDone = 1, kBudget = 2, kOverflow = 3 };
+enum : int {
+  kStDecomposed = 0, kStBounded, kStLeaves, kStPrunedPop, kStPrunedIns,
+  kStImproved, kStMaxPool, kStRounds, kStSubset, kStFallback, kStPairs, kStWork
+};
+constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
+constexpr int kSelWin = 1024;
+
+struct Ctrl {
+  int status;
+  int stack_size;
+  int inc;
+  int n_parents;
+  int n_children;
+  int push_base;
+  int push_total;
+  int best_child;  // child index whose leaf set the incumbent this round
+  int have_perm;
+  int pad[7];
+  long long stats[WSBB_NSTATS];
+  uint8_t best_perm[kMaxJobs];
+};
+
+struct SArgs {
+  int n, m, P, CAP, PCAP, CCAP, pool_m1, max_batches;
+  long long budget;
+  Ctrl* ctrl;
+  uint8_t* s_perm;  // [n][CAP]
+  int* s_l1;
+  int* s_l2;
+  int* s_cost;
+  uint8_t* p_perm;  // [n][PCAP]
+  int* p_l1;
+  int* p_l2;
+  int* p_cost;
+  int* p_sel;       // stack index of parent
+  int* p_cnt;       // children of parent
+  int* p_off;       // [PCAP+1] first child
+  int* p_side;
+  int* p_nch;       // children on the chosen side
+  int* p_leafmin;
+  int* p_leafmax;   // cost of the leaf a simple parent inserts first
+  int* p_leafwin;
+  int* p_incb;
+  int* p_kept;
+  int* p_koff;
+  int* c_desc;
+  int* c_bound;
+  int* c_rank;
+  int* c_scost;
+};
+
+__device__ __forceinline__ int wave_incl_sum(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// exclusive block scan over in[0..L) with op/identity; returns the total
+template <class Op>
+__device__ int block_scan_excl(const int* in, int* out, int L, int ident, Op op,
+                               int* s_tmp) {
+  const int B = blockDim.x, t = threadIdx.x;
+  const int seg = (L + B - 1) / B;
+  const int lo = min(t * seg, L), hi = min(lo + seg, L);
+  int loc = ident;
+  for (int i = lo; i < hi; ++i) loc = op(loc, in[i]);
+  s_tmp[t] = loc;
+  __syncthreads();
+  for (int d = 1; d < B; d <<= 1) {
+    const int o = (t >= d) ? s_tmp[t - d] : ident;
+    __syncthreads();
+    s_tmp[t] = op(s_tmp[t], o);
+    __syncthreads();
+  }
+  int run = (t == 0) ? ident : s_tmp[t - 1];
+  const int total = s_tmp[B - 1];
+  for (int i = lo; i < hi; ++i) {
+    const int x = in[i];
+    out[i] = run;
+    run = op(run, x);
+  }
+  __syncthreads();
+  return total;
+}
+
+struct OpAdd {
+  __device__ int operator()(int a, int b) const { return a + b; }
+};
+struct OpMin {
+  __device__ int operator()(int a, int b) const { return a < b ? a : b; }
+};
+
+// ---- select: one block; wave 0 runs the greedy sub-batch packing ---------
+__global__ __launch_bounds__(256) void k_select(SArgs S) {
+  __shared__ int s_w[kSelWin];
+  __shared__ int s_c[kSelWin];
+  __shared__ int s_state[4];
+  __shared__ int s_tmp[256];
+  Ctrl* C = S.ctrl;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (C->status != kRunning) {
+    if (tid == 0) {
+      C->n_parents = 0;
+      C->n_children = 0;
+    }
+    return;
+  }
+  const int top = C->stack_size;
+  const int n = S.n;
+  if (top == 0) {
+    if (tid == 0) {
+      C->status = kDone;
+      C->n_parents = 0;
+      C->n_children = 0;
+    }
+    return;
+  }
+  const int inc = C->inc;
+  // wave-0 state (identical in all its lanes)
+  int size1 = 0, nb = 0, np = 0, pruned = 0, stop = 0;
+  int consumed = 0;
+  for (;;) {
+    const int win = min(kSelWin, top - consumed);
+    if (win <= 0) break;
+    for (int t = tid; t < win; t += blockDim.x) {
+      const int si = top - 1 - (consumed + t);
+      const int l1 = S.s_l1[si], l2 = S.s_l2[si];
+      const int depth = (l1 + 1) + (n - l2);
+      const int u = n - depth;
+      s_w[t] = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
+      s_c[t] = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
+    }
+    __syncthreads();
+    if (wave == 0) {
+      int used = win;  // candidates of this window consumed
+      for (int base = 0; base < win && !stop; base += 64) {
+        const int t = base + lane;
+        const bool valid = t < win;
+        const int W = valid ? s_w[t] : 0;
+        const int incl = wave_incl_sum(W, lane);
+        int start = 0;
+        for (;;) {
+          const int excl_start = (start == 0) ? 0 : __shfl(incl, start - 1, 64);
+          const bool brk = valid && lane >= start && W > 0 && (size1 + incl - excl_start >= S.pool_m1);
+          const unsigned long long bm = __ballot(brk);
+          const int end = bm ? (__ffsll(static_cast<long long>(bm)) - 1) : 64;
+          const bool in = valid && lane >= start && lane < end;
+          const bool tk = in && W > 0;
+          const unsigned long long tm = __ballot(tk);
+          const int before = __popcll(tm & ((1ull << lane) - 1ull));
+          if (tk && np + before < S.PCAP) {
+            S.p_sel[np + before] = top - 1 - (consumed + t);
+            S.p_cnt[np + before] = s_c[t];
+          }
+          np += __popcll(tm);
+          pruned += __popcll(__ballot(in && W == 0));
+          const int last = (end == 64) ? 63 : (end > start ? end - 1 : start);
+          const int upto = (end > start) ? __shfl(incl, last, 64) : excl_start;
+          size1 += upto - excl_start;
+          if (end == 64) break;
+          // the sub-batch closes in front of lane `end` (that parent is
+          // re-inserted and opens the next sub-batch)
+          ++nb;
+          if (nb == S.max_batches) {
+            stop = 1;
+            used = base + end;
+            break;
+          }
+          size1 = 0;
+          start = end;
+        }
+      }
+      if (lane == 0) {
+        s_state[0] = used;
+        s_state[1] = stop;
+      }
+    }
+    __syncthreads();
+    consumed += s_state[0];
+    const int st = s_state[1];
+    __syncthreads();
+    if (st) break;
+  }
+  if (wave == 0 && lane == 0) {
+    s_state[2] = np;
+    s_state[3] = pruned;
+  }
+  __syncthreads();
+  np = s_state[2];
+  pruned = s_state[3];
+  if (np > S.PCAP) {
+    if (tid == 0) C->status = kOverflow;
+    return;
+  }
+  const int nc = block_scan_excl(S.p_cnt, S.p_off, np, 0, OpAdd(), s_tmp);
+  if (tid == 0) {
+    S.p_off[np] = nc;
+    C->stats[kStRounds] += 1;
+    C->stats[kStPrunedPop] += pruned;
+    C->stack_size = top - consumed;
+    if (S.budget > 0 && C->stats[kStDecomposed] + np > S.budget) {
+      C->status = kBudget;
+      C->n_parents = 0;
+      C->n_children = 0;
+    } else if (np == 0) {
+      C->status = kDone;
+      C->n_parents = 0;
+      C->n_children = 0;
+    } else {
+      C->n_parents = np;
+      C->n_children = nc;
+      C->stats[kStDecomposed] += np;
+      C->stats[kStBounded] += nc;
+    }
+    C->best_child = -1;
+  }
+}
+
+// ---- expand: parents out of the stack, child descriptors ------------------
+__global__ __launch_bounds__(256) void k_expand(SArgs S) {
+  const Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  const int n = S.n, m = S.m, P = S.P;
+  long long work = 0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    const int si = S.p_sel[p];
+    for (int i = 0; i < n; ++i)
+      S.p_perm[static_cast<size_t>(i) * S.PCAP + p] = S.s_perm[static_cast<size_t>(i) * S.CAP + si];
+    const int l1 = S.s_l1[si], l2 = S.s_l2[si];
+    S.p_l1[p] = l1;
+    S.p_l2[p] = l2;
+    S.p_cost[p] = S.s_cost[si];
+    int off = S.p_off[p];
+    const int depth = (l1 + 1) + (n - l2);
+    const int tag = p << 9;
+    // algorithmic bound work of the children: per child with prefix d1,
+    // suffix d2 and u free jobs, P(n + 4u + 4) + 2m d1 + m(m+1) d2 int ops
+    const long long d1 = l1 + 1, d2 = n - l2, u = n - depth;
+    const long long mm1 = static_cast<long long>(m) * (m + 1);
+    if (depth == n - 2) {  // simple parent: two leaves, BEGIN side
+      S.c_desc[off] = tag | (l1 + 1);
+      S.c_desc[off + 1] = tag | (l1 + 2);
+      work += 2 * (P * (n + 4LL) + 2LL * m * (d1 + 2) + mm1 * d2);
+    } else {
+      for (int side = 0; side < 2; ++side)
+        for (int j = l1 + 1; j < l2; ++j) S.c_desc[off++] = tag | (side << 8) | j;
+      const long long base = P * (n + 4 * (u - 1) + 4);
+      work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
+  if ((threadIdx.x & 63) == 0 && work)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStWork]),
+              static_cast<unsigned long long>(work));
+}
+
+// ---- rank: one wave per parent -------------------------------------------
+__global__ __launch_bounds__(256) void k_rank(SArgs S) {
+  __shared__ int s_cost[4][kMaxJobs];
+  __shared__ int s_job[4][kMaxJobs];
+  const Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = S.n;
+  long long leaves = 0;
+  for (int pb = blockIdx.x * 4; pb < np; pb += gridDim.x * 4) {
+    const int p = pb + wave;
+    const bool live = p < np;
+    int off = 0, cnt = 0, u = 0, simple = 0, side = 0, nch = 0;
+    if (live) {
+      off = S.p_off[p];
+      cnt = S.p_off[p + 1] - off;
+      const int l1 = S.p_l1[p], l2 = S.p_l2[p];
+      u = l2 - l1 - 1;
+      simple = ((l1 + 1) + (n - l2)) == n - 2;
+      int c0 = 0, c1 = 0, s0 = 0, s1 = 0;
+      for (int c = lane; c < cnt; c += 64) {
+        const int b = S.c_bound[off + c];
+        const int sd = simple ? 0 : (c >= u);
+        const int big = b >= kSideBest;
+        if (sd == 0) {
+          c0 += big;
+          s0 += big ? 0 : b;
+        } else {
+          c1 += big;
+          s1 += big ? 0 : b;
+        }
+      }
+      for (int d = 32; d >= 1; d >>= 1) {
+        c0 += __shfl_xor(c0, d, 64);
+        c1 += __shfl_xor(c1, d, 64);
+        s0 += __shfl_xor(s0, d, 64);
+        s1 += __shfl_xor(s1, d, 64);
+      }
+      side = (c0 > c1) ? 0 : (c1 > c0) ? 1 : (s0 > s1) ? 0 : 1;
+      nch = simple ? (side == 0 ? 2 : 0) : u;
+      const int first = simple ? 0 : side * u;
+      for (int i = lane; i < nch; i += 64) {
+        const int c = first + i;
+        const int j = S.c_desc[off + c] & 255;
+        s_cost[wave][i] = S.c_bound[off + c];
+        s_job[wave][i] = S.p_perm[static_cast<size_t>(j) * S.PCAP + p];
+      }
+    }
+    __syncthreads();
+    if (live) {
+      const int first = simple ? 0 : side * u;
+      int lmin = INT_MAX, lwin = -1, lwjob = -1, lmax = INT_MIN;
+      for (int c = lane; c < cnt; c += 64) {
+        const int i = c - first;
+        if (i < 0 || i >= nch) {
+          S.c_rank[off + c] = -1;
+          continue;
+        }
+        const int cc = s_cost[wave][i], jj = s_job[wave][i];
+        int rk = 0;
+        for (int k = 0; k < nch; ++k) {
+          const int ck = s_cost[wave][k];
+          rk += (ck < cc) || (ck == cc && s_job[wave][k] < jj);
+        }
+        S.c_rank[off + c] = rk;
+        S.c_scost[off + rk] = cc;
+        if (simple) {
+          // leaves are inserted worst first (cost desc, job desc) and only a
+          // strictly better one replaces the incumbent: among equal minima
+          // the one with the larger job id wins
+          if (cc < lmin || (cc == lmin && jj > lwjob)) {
+            lmin = cc;
+            lwin = off + c;
+            lwjob = jj;
+          }
+          lmax = max(lmax, cc);
+        }
+      }
+      if (simple) {
+        for (int d = 32; d >= 1; d >>= 1) {
+          const int om = __shfl_xor(lmin, d, 64), ow = __shfl_xor(lwin, d, 64),
+                    oj = __shfl_xor(lwjob, d, 64);
+          if (om < lmin || (om == lmin && oj > lwjob)) {
+            lmin = om;
+            lwin = ow;
+            lwjob = oj;
+          }
+          lmax = max(lmax, __shfl_xor(lmax, d, 64));
+        }
+      }
+      if (lane == 0) {
+        S.p_side[p] = side;
+        S.p_nch[p] = nch;
+        S.p_leafmin[p] = simple ? lmin : INT_MAX;
+        S.p_leafmax[p] = (simple && nch > 0) ? lmax : INT_MAX;
+        S.p_leafwin[p] = simple ? lwin : -1;
+        if (simple) leaves += nch;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && leaves) atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStLeaves]),
+                                     static_cast<unsigned long long>(leaves));
+}
+
+// ---- incumbent seen by each parent (running min of earlier leaves) --------
+__global__ __launch_bounds__(1024) void k_incumbent(SArgs S) {
+  __shared__ int s_tmp[1024];
+  __shared__ int s_last;
+  Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  if (np == 0) return;
+  const int inc0 = C->inc;
+  block_scan_excl(S.p_leafmin, S.p_incb, np, INT_MAX, OpMin(), s_tmp);
+  if (threadIdx.x == 0) s_last = -1;
+  __syncthreads();
+  int improved = 0, last = -1;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const int ib = min(inc0, S.p_incb[p]);
+    S.p_incb[p] = ib;
+    const int lmin = S.p_leafmin[p];
+    if (lmin < ib) {
+      // two leaves, inserted worst first: the first may improve, then the
+      // second only if strictly better still
+      const int lmax = S.p_leafmax[p];
+      improved += (lmax < ib) + (lmin < min(ib, lmax));
+      last = max(last, p);
+    }
+  }
+  atomicMax(&s_last, last);
+  if (improved) atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStImproved]),
+                          static_cast<unsigned long long>(improved));
+  __syncthreads();
+  if (threadIdx.x == 0 && s_last >= 0) {
+    const int p = s_last;
+    C->inc = S.p_leafmin[p];
+    C->best_child = S.p_leafwin[p];
+  }
+}
+
+// ---- kept children per parent ---------------------------------------------
+__global__ __launch_bounds__(256) void k_keep(SArgs S) {
+  const Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  const int n = S.n;
+  long long pruned = 0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    const int l1 = S.p_l1[p], l2 = S.p_l2[p];
+    const bool simple = ((l1 + 1) + (n - l2)) == n - 2;
+    int kept = 0;
+    if (!simple) {
+      const int off = S.p_off[p], nch = S.p_nch[p], lim = S.p_incb[p];
+      int lo = 0, hi = nch;  // first rank whose bound is >= lim
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (S.c_scost[off + mid] < lim) lo = mid + 1;
+        else hi = mid;
+      }
+      kept = lo;
+      pruned += nch - kept;
+    }
+    S.p_kept[p] = kept;
+  }
+  for (int d = 32; d >= 1; d >>= 1) pruned += __shfl_xor(pruned, d, 64);
+  if ((threadIdx.x & 63) == 0 && pruned)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStPrunedIns]),
+              static_cast<unsigned long long>(pruned));
+}
+
+// ---- survivor offsets, capacity check, best permutation -------------------
+__global__ __launch_bounds__(1024) void k_offsets(SArgs S) {
+  __shared__ int s_tmp[1024];
+  Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  if (np == 0) return;
+  const int total = block_scan_excl(S.p_kept, S.p_koff, np, 0, OpAdd(), s_tmp);
+  if (threadIdx.x == 0) {
+    const int base = C->stack_size;
+    C->push_base = base;
+    C->push_total = total;
+    if (static_cast<long long>(base) + total > S.CAP) {
+      C->status = kOverflow;
+    } else {
+      C->stack_size = base + total;
+      if (base + total > C->stats[kStMaxPool]) C->stats[kStMaxPool] = base + total;
+    }
+  }
+  // the leaf that set the incumbent: parent permutation with its swap
+  const int bc = C->best_child;
+  if (bc >= 0 && threadIdx.x < S.n) {
+    const int d = S.c_desc[bc];
+    const int p = d >> 9, side = (d >> 8) & 1, j = d & 255;
+    const int sw0 = side == 0 ? S.p_l1[p] + 1 : S.p_l2[p] - 1;
+    const int i = threadIdx.x;
+    const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
+    C->best_perm[i] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
+    if (i == 0) C->have_perm = 1;
+  }
+}
+
+// ---- push survivors -------------------------------------------------------
+__global__ __launch_bounds__(256) void k_push(SArgs S) {
+  const Ctrl* C = S.ctrl;
+  if (C->status == kOverflow) return;
+  const int nc = C->n_children;
+  const int n = S.n;
+  const int base = C->push_base, total = C->push_total;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    const int rk = S.c_rank[c];
+    if (rk < 0) continue;
+    const int d = S.c_desc[c];
+    const int p = d >> 9;
+    if (rk >= S.p_kept[p]) continue;
+    const int side = (d >> 8) & 1, j = d & 255;
+    const int slot = base + total - 1 - (S.p_koff[p] + rk);
+    const int pl1 = S.p_l1[p], pl2 = S.p_l2[p];
+    int l1 = pl1 + (side == 0), l2 = pl2 - (side == 1);
+    if (l2 - l1 == 2) l1 += 1;
+    const int sw0 = side == 0 ? pl1 + 1 : pl2 - 1;
+    for (int i = 0; i < n; ++i) {
+      const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
+      S.s_perm[static_cast<size_t>(i) * S.CAP + slot] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
+    }
+    S.s_l1[slot] = l1;
+    S.s_l2[slot] = l2;
+    S.s_cost[slot] = S.c_bound[c];
+  }
+}
+
+__global__ void k_init(SArgs S, int ub0) {
+  Ctrl* C = S.ctrl;
+  const int n = S.n;
+  const int i = threadIdx.x;
+  if (i < n) S.s_perm[static_cast<size_t>(i) * S.CAP] = static_cast<uint8_t>(i);
+  if (i < kMaxJobs) C->best_perm[i] = 0;
+  if (i < WSBB_NSTATS) C->stats[i] = 0;
+  if (i == 0) {
+    S.s_l1[0] = -1;
+    S.s_l2[0] = n;
+    S.s_cost[0] = 0;
+    C->status = kRunning;
+    C->stack_size = 1;
+    C->inc = ub0;
+    C->n_parents = C->n_children = 0;
+    C->best_child = -1;
+    C->have_perm = 0;
+  }
+}
+
+}  // namespace
+
+struct SearchState {
+  int n = 0, CAP = 0, PCAP = 0, CCAP = 0;
+  DevBuf mem;
+  SArgs args{};
+  Ctrl* host_ctrl = nullptr;  // pinned
+  // optional timing of every bound launch (pairs of events per round)
+  std::vector<hipEvent_t> ev;
+  double bound_ms = 0;
+  long long bound_launches = 0;
+};
+
+void destroy_search(wsbb_ctx* ctx) {
+  if (!ctx->search) return;
+  for (hipEvent_t e : ctx->search->ev) (void)hipEventDestroy(e);
+  ctx->search->mem.release();
+  if (ctx->search->host_ctrl) (void)hipHostFree(ctx->search->host_ctrl);
+  delete ctx->search;
+  ctx->search = nullptr;
+}
+
+static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_nodes) {
+  const int n = ctx->T.n;
+  const int PCAP = max_batches * ((pool_size - 2) / 4 + 1);
+  const int CCAP = max_batches * pool_size;
+  SearchState* S = ctx->search;
+  if (S && S->PCAP >= PCAP && S->CCAP >= CCAP && S->CAP == cap_nodes) return WSBB_OK;
+  destroy_search(ctx);
+  S = new SearchState();
+  ctx->search = S;
+  S->n = n;
+  S->CAP = cap_nodes;
+  S->PCAP = PCAP;
+  S->CCAP = CCAP;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += al(bytes);
+    return o;
+  };
+  const size_t o_ctrl = take(sizeof(Ctrl));
+  const size_t o_sperm = take(size_t(n) * cap_nodes);
+  const size_t o_sl1 = take(4 * size_t(cap_nodes));
+  const size_t o_sl2 = take(4 * size_t(cap_nodes));
+  const size_t o_scost = take(4 * size_t(cap_nodes));
+  const size_t o_pperm = take(size_t(n) * PCAP);
+  size_t o_pint[14];
+  for (int i = 0; i < 14; ++i) o_pint[i] = take(4 * size_t(PCAP + 1));
+  size_t o_cint[4];
+  for (int i = 0; i < 4; ++i) o_cint[i] = take(4 * size_t(CCAP));
+  hipError_t e = S->mem.reserve(off);
+  if (e != hipSuccess) {
+    destroy_search(ctx);
+    hip_fail(e, "hipMalloc(search pool)");
+    return WSBB_ENOMEM;
+  }
+  e = hipHostMalloc(reinterpret_cast<void**>(&S->host_ctrl), sizeof(Ctrl), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    destroy_search(ctx);
+    return hip_fail(e, "hipHostMalloc(ctrl)");
+  }
+  SArgs& A = S->args;
+  A.n = n;
+  A.m = ctx->T.m;
+  A.P = ctx->T.P;
+  A.CAP = cap_nodes;
+  A.PCAP = PCAP;
+  A.CCAP = CCAP;
+  A.ctrl = S->mem.as<Ctrl>(o_ctrl);
+  A.s_perm = S->mem.as<uint8_t>(o_sperm);
+  A.s_l1 = S->mem.as<int>(o_sl1);
+  A.s_l2 = S->mem.as<int>(o_sl2);
+  A.s_cost = S->mem.as<int>(o_scost);
+  A.p_perm = S->mem.as<uint8_t>(o_pperm);
+  int** pints[14] = {&A.p_l1,      &A.p_l2,      &A.p_cost,    &A.p_sel,  &A.p_cnt,
+                     &A.p_off,     &A.p_side,    &A.p_nch,     &A.p_leafmin, &A.p_leafmax,
+                     &A.p_leafwin, &A.p_incb,    &A.p_kept,    &A.p_koff};
+  for (int i = 0; i < 14; ++i) *pints[i] = S->mem.as<int>(o_pint[i]);
+  int** cints[4] = {&A.c_desc, &A.c_bound, &A.c_rank, &A.c_scost};
+  for (int i = 0; i < 4; ++i) *cints[i] = S->mem.as<int>(o_cint[i]);
+  return WSBB_OK;
+}
+
+static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent_t ev0,
+                               hipEvent_t ev1) {
+  hipStream_t st = ctx->stream;
+  const int pblocks = (A.PCAP + 255) / 256;
+  const int cblocks = (A.CCAP + 255) / 256;
+  const int rblocks = (A.PCAP + 3) / 4;
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_expand, dim3(pblocks), dim3(256), 0, st, A);
+  ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
+  BoundArgs B{};
+  B.N = A.CCAP;
+  B.Ndev = &A.ctrl->n_children;
+  B.kind = kind;
+  B.best = -1;
+  B.groups = ctx->groups_for(A.CCAP);
+  B.out = A.c_bound;
+  if (ev0) (void)hipEventRecord(ev0, st);
+  hipError_t e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
+  if (e != hipSuccess) return e;
+  if (ev1) (void)hipEventRecord(ev1, st);
+  hipLaunchKernelGGL(k_rank, dim3(rblocks < 4096 ? rblocks : 4096), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_incumbent, dim3(1), dim3(1024), 0, st, A);
+  hipLaunchKernelGGL(k_keep, dim3(pblocks), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_offsets, dim3(1), dim3(1024), 0, st, A);
+  hipLaunchKernelGGL(k_push, dim3(cblocks), dim3(256), 0, st, A);
+  return hipGetLastError();
+}
+
+int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
+          int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
+          int cap_nodes, int rounds_per_sync) {
+  const int n = ctx->T.n;
+  if (pool_size - 1 <= 2 * n || max_batches < 1 || max_batches > 64 ||
+      static_cast<long long>(pool_size) * max_batches > (1 << 23)) {
+    set_error("pool_size must exceed 2n+1 and pool_size*max_batches must be <= 2^23");
+    return WSBB_EINVAL;
+  }
+  if (kind != WSBB_LB2 && kind != WSBB_LB1) {
+    set_error("search kind must be WSBB_LB2 or WSBB_LB1");
+    return WSBB_EINVAL;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  int rc = ensure_search(ctx, pool_size, max_batches, cap_nodes);
+  if (rc) return rc;
+  SearchState* S = ctx->search;
+  SArgs A = S->args;
+  A.pool_m1 = pool_size - 1;
+  A.max_batches = max_batches;
+  A.budget = budget;
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");
+  const bool timing = ctx->profile_bound;
+  if (timing && static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
+    while (static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
+      hipEvent_t ev;
+      if ((e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+      S->ev.push_back(ev);
+    }
+  }
+  S->bound_ms = 0;
+  S->bound_launches = 0;
+  for (;;) {
+    for (int r = 0; r < rounds_per_sync; ++r)
+      if ((e = launch_round(ctx, A, kind, timing ? S->ev[2 * r] : nullptr,
+                            timing ? S->ev[2 * r + 1] : nullptr)) != hipSuccess)
+        return hip_fail(e, "round launch");
+    e = hipMemcpyAsync(S->host_ctrl, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "search sync");
+    if (timing)
+      for (int r = 0; r < rounds_per_sync; ++r) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, S->ev[2 * r], S->ev[2 * r + 1]) == hipSuccess) {
+          S->bound_ms += t;
+          ++S->bound_launches;
+        }
+      }
+    if (S->host_ctrl->status != kRunning) break;
+  }
+  const Ctrl& H = *S->host_ctrl;
+  if (best) *best = H.inc;
+  if (best_perm) {
+    for (int i = 0; i < n; ++i) best_perm[i] = H.have_perm ? H.best_perm[i] : -1;
+  }
+  if (stats)
+    for (int i = 0; i < WSBB_NSTATS; ++i) stats[i] = H.stats[i];
+  if (H.status == kOverflow) {
+    set_error("device node pool overflow; raise the pool capacity");
+    return WSBB_EOVERFLOW;
+  }
+  if (H.status == kBudget) {
+    set_error("node budget exhausted");
+    return WSBB_EBUDGET;
+  }
+  return WSBB_OK;
+}
+
+}  // namespace wsbb
+
+extern "C" int wsbb_search_profile(wsbb_ctx* ctx, int enable, double* bound_ms,
+                                   int64_t* bound_launches) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (enable >= 0) ctx->profile_bound = enable != 0;
+  if (bound_ms) *bound_ms = ctx->search ? ctx->search->bound_ms : 0.0;
+  if (bound_launches) *bound_launches = ctx->search ? ctx->search->bound_launches : 0;
+  return WSBB_OK;
+}
+
+extern "C" int wsbb_solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches,
+                          int32_t ub0, int64_t budget, int32_t* best,
+                          int32_t* best_perm, int64_t* stats) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  return wsbb::solve(ctx, kind, pool_size, max_batches, ub0, budget, best, best_perm,
+                     stats, ctx->pool_capacity, 4);
+}
