@@ -66,13 +66,13 @@ def bench_solve(wsbb_amd, steps, warmup):
     return dict(elapsed=t1 - t0, best=best, stats=stats, **tot)
 
 
-def bench_batched(wsbb_amd, reps=20):
+def bench_batched(wsbb_amd, reps=20, autotune=True):
     p = wsbb_amd.taillard(51)
     m, n = p.shape
     N = 1 << 20
     permT, l1, l2 = wsbb_amd.random_nodes(n, N, 0x5EED)
     eng = wsbb_amd.Engine(p)
-    tuned = eng.autotune(N, reps=3)
+    tuned = eng.autotune(N, reps=3) if autotune else []
     dp, d1, d2, do = eng.malloc(permT.nbytes), eng.malloc(4 * N), eng.malloc(4 * N), eng.malloc(4 * N)
     eng.h2d(dp, permT)
     eng.h2d(d1, l1)
@@ -113,8 +113,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("all", "solve", "batched", "copy"), default="all",
+                    help="solve = ta011 time-to-optimum only; batched = 50x20 launches only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build targets a single MI355X "
@@ -122,11 +124,30 @@ def main():
         sys.exit(2)
 
     import wsbb_amd
+    if args.workload == "copy":  # calibration of the hardware byte counters
+        eng = wsbb_amd.Engine(wsbb_amd.taillard(1))
+        nbytes = 256 << 20
+        src, dst = eng.malloc(nbytes), eng.malloc(nbytes)
+        eng.timer_start()
+        for _ in range(args.steps):
+            eng.copy_device(dst, src, nbytes)
+        ms = eng.timer_stop() / args.steps
+        eng.free(src)
+        eng.free(dst)
+        print(json.dumps({"workload": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "ms": ms, "GBps": 2 * nbytes / ms / 1e6}), flush=True)
+        return
+    if args.workload == "batched":
+        bt = bench_batched(wsbb_amd, reps=args.steps, autotune=not args.no_autotune)
+        print(json.dumps({"workload": "batched", "ms_per_launch": bt["ms"],
+                          "nodes_per_s": bt["N"] / (bt["ms"] / 1e3)}), flush=True)
+        return
     sv = bench_solve(wsbb_amd, args.steps, args.warmup)
-    bt = None if args.no_batched else bench_batched(wsbb_amd)
+    bt = None if args.workload == "solve" else bench_batched(
+        wsbb_amd, autotune=not args.no_autotune)
     cpu_solve = cpu_b = None
-    if not args.no_cpu:
-        cpu_solve, cpu_b = cpu_baselines(bt) if bt else (None, None)
+    if not args.no_cpu and bt:
+        cpu_solve, cpu_b = cpu_baselines(bt)
 
     value = sv["bounded"] / sv["elapsed"]
     bound_s = sv["bound_ms"] / 1e3

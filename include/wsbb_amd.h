@@ -117,6 +117,11 @@ int wsbb_free(wsbb_ctx *ctx, void *d_ptr);
 int wsbb_h2d(wsbb_ctx *ctx, void *d_dst, const void *src, size_t bytes);
 int wsbb_d2h(wsbb_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 int wsbb_synchronize(wsbb_ctx *ctx);
+/* Plain streaming device-to-device copy (16-byte aligned), asynchronous on
+ * the context's stream.  A measurement helper with no counterpart in the
+ * reference: it calibrates hardware byte counters against a known size. */
+int wsbb_copy_device(wsbb_ctx *ctx, void *d_dst, const void *d_src,
+                     size_t bytes);
 /* HIP events recorded on the context's stream around a timed region */
 int wsbb_timer_start(wsbb_ctx *ctx);
 int wsbb_timer_stop(wsbb_ctx *ctx, float *ms);
@@ -156,6 +161,44 @@ int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
                         int64_t *bound_launches);
 /* Device node-pool capacity in nodes (default 2^24) for later solves. */
 int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
+
+/* ---- learned machine-pair selector -------------------------------------------
+ * No counterpart in the reference (its pair order ordoSomme is fixed,
+ * bound_flowshop.cpp:130 is commented out).  A two-layer MLP (5m features:
+ * per machine head, tail, remaining work, head+work, work+tail; 128 hidden
+ * ReLU units; one score per machine pair), BF16 weights with FP32
+ * accumulation, trained on records (node features, LB2-maximising pair)
+ * logged by plain solves.  With kind WSBB_LEARNED the search bounds each
+ * 64-child wavefront tile over the top-k pairs of its summed predicted
+ * probabilities; a child whose subset bound lies in [incumbent - margin,
+ * incumbent) gets the remaining pairs too (full LB2).  Any subset bound is
+ * a valid lower bound, so the proven optimum is unchanged.
+ * Record layout: feat [5m][count] (float, scaled by the instance's path
+ * bound), label [count] (pair index, -1 for leaves). */
+int wsbb_selector_init(wsbb_ctx *ctx, uint64_t seed);
+/* set = 1 uploads, 0 downloads the FP32 master weights: W1 [5m][128],
+ * b1 [128], W2 [128][P], b2 [P]; NULL pointers are skipped. */
+int wsbb_selector_params(wsbb_ctx *ctx, int set, float *W1, float *b1,
+                         float *W2, float *b2);
+int wsbb_selector_config(wsbb_ctx *ctx, int topk, int margin);
+/* log every stride-th child of later plain solves into a device store of
+ * `capacity` records (stride 0 stops logging); clears the store */
+int wsbb_selector_log(wsbb_ctx *ctx, int stride, int capacity);
+int wsbb_selector_records(wsbb_ctx *ctx, float *feat, int32_t *label,
+                          int *count);
+int wsbb_selector_set_records(wsbb_ctx *ctx, const float *feat,
+                              const int32_t *label, int count);
+/* one forward/backward pass (softmax cross-entropy, mean over labelled
+ * rows) over records idx[0..B): loss and gradients of the masters */
+int wsbb_selector_grad(wsbb_ctx *ctx, const int32_t *idx, int B, float *loss,
+                       float *gW1, float *gb1, float *gW2, float *gb2);
+/* Adam training on the stored records; epoch_loss [epochs] may be NULL */
+int wsbb_selector_train(wsbb_ctx *ctx, int epochs, int batch, float lr,
+                        uint64_t seed, float *epoch_loss);
+/* MFMA inference: scores [P][N] logits (may be NULL) and the per-tile
+ * top-k masks tile_sel [ceil(N/64)][ceil(P/32)] (may be NULL) */
+int wsbb_selector_infer(wsbb_ctx *ctx, const float *feat, int N,
+                        float *scores, uint32_t *tile_sel);
 
 #ifdef __cplusplus
 }

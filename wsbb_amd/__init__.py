@@ -68,6 +68,18 @@ def _load():
         "wsbb_search_profile": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _i64p],
                                 ctypes.c_int),
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
+        "wsbb_copy_device": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "wsbb_selector_init": ([_vp, ctypes.c_uint64], ctypes.c_int),
+        "wsbb_selector_params": ([_vp, ctypes.c_int, _f32p, _f32p, _f32p, _f32p], ctypes.c_int),
+        "wsbb_selector_config": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_selector_log": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_selector_records": ([_vp, _f32p, _i32p, _i32p], ctypes.c_int),
+        "wsbb_selector_set_records": ([_vp, _f32p, _i32p, ctypes.c_int], ctypes.c_int),
+        "wsbb_selector_grad": ([_vp, _i32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p, _f32p],
+                               ctypes.c_int),
+        "wsbb_selector_train": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                 _f32p], ctypes.c_int),
+        "wsbb_selector_infer": ([_vp, _f32p, ctypes.c_int, _f32p, _u32p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -212,6 +224,9 @@ class Engine:
     def bound_device(self, d_perm, d_l1, d_l2, N, d_out, kind=LB2):
         _check(lib.wsbb_bound_device(self.h, d_perm, d_l1, d_l2, N, kind, d_out), "bound_device")
 
+    def copy_device(self, d_dst, d_src, nbytes):
+        _check(lib.wsbb_copy_device(self.h, d_dst, d_src, nbytes), "copy_device")
+
     def synchronize(self):
         _check(lib.wsbb_synchronize(self.h), "synchronize")
 
@@ -245,6 +260,73 @@ class Engine:
 
     def set_pool_capacity(self, nodes):
         _check(lib.wsbb_set_pool_capacity(self.h, nodes), "set_pool_capacity")
+
+    # ---- learned pair selector ----
+    HIDDEN = 128
+
+    def selector_init(self, seed=1):
+        _check(lib.wsbb_selector_init(self.h, seed), "selector_init")
+
+    def selector_params(self, params=None):
+        """Download the FP32 masters, or upload `params` (dict W1, b1, W2, b2)."""
+        F, H, P = 5 * self.m, self.HIDDEN, self.P
+        if params is None:
+            out = dict(W1=np.zeros((F, H), np.float32), b1=np.zeros(H, np.float32),
+                       W2=np.zeros((H, P), np.float32), b2=np.zeros(P, np.float32))
+            _check(lib.wsbb_selector_params(self.h, 0, *[_p(out[k], _f32p) for k in
+                                                         ("W1", "b1", "W2", "b2")]), "params")
+            return out
+        arrs = [np.ascontiguousarray(params[k], np.float32) for k in ("W1", "b1", "W2", "b2")]
+        _check(lib.wsbb_selector_params(self.h, 1, *[_p(a, _f32p) for a in arrs]), "params")
+
+    def selector_config(self, topk=16, margin=0):
+        _check(lib.wsbb_selector_config(self.h, topk, margin), "selector_config")
+
+    def selector_log(self, stride, capacity):
+        _check(lib.wsbb_selector_log(self.h, stride, capacity), "selector_log")
+
+    def selector_records(self):
+        cnt = ctypes.c_int32()
+        _check(lib.wsbb_selector_records(self.h, None, None, ctypes.byref(cnt)), "records")
+        c = cnt.value
+        feat = np.zeros((5 * self.m, c), np.float32)
+        lab = np.zeros(c, np.int32)
+        if c:
+            _check(lib.wsbb_selector_records(self.h, _p(feat, _f32p), _p(lab), ctypes.byref(cnt)),
+                   "records")
+        return feat, lab
+
+    def selector_set_records(self, feat, label):
+        feat = np.ascontiguousarray(feat, np.float32)
+        label = np.ascontiguousarray(label, np.int32)
+        _check(lib.wsbb_selector_set_records(self.h, _p(feat, _f32p), _p(label), label.shape[0]),
+               "set_records")
+
+    def selector_grad(self, idx):
+        idx = np.ascontiguousarray(idx, np.int32)
+        F, H, P = 5 * self.m, self.HIDDEN, self.P
+        g = dict(W1=np.zeros((F, H), np.float32), b1=np.zeros(H, np.float32),
+                 W2=np.zeros((H, P), np.float32), b2=np.zeros(P, np.float32))
+        loss = ctypes.c_float()
+        _check(lib.wsbb_selector_grad(self.h, _p(idx), idx.shape[0], ctypes.byref(loss),
+                                      *[_p(g[k], _f32p) for k in ("W1", "b1", "W2", "b2")]),
+               "grad")
+        return loss.value, g
+
+    def selector_train(self, epochs=5, batch=512, lr=1e-3, seed=1):
+        losses = np.zeros(epochs, np.float32)
+        _check(lib.wsbb_selector_train(self.h, epochs, batch, lr, seed, _p(losses, _f32p)),
+               "train")
+        return losses
+
+    def selector_infer(self, feat):
+        feat = np.ascontiguousarray(feat, np.float32)
+        N = feat.shape[1]
+        scores = np.zeros((self.P, N), np.float32)
+        tsel = np.zeros(((N + 63) // 64, (self.P + 31) // 32), np.uint32)
+        _check(lib.wsbb_selector_infer(self.h, _p(feat, _f32p), N, _p(scores, _f32p),
+                                       _p(tsel, _u32p)), "infer")
+        return scores, tsel
 
     def solve(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
         best = ctypes.c_int32()

@@ -2,19 +2,22 @@
 //
 // One lane bounds one node.  All lanes of a wavefront walk the same machine
 // pair and the same Johnson step at the same time, so every per-step constant
-// (job id, p_a, p_b, lag) is wave-uniform and arrives through scalar loads
-// into SGPRs; the per-lane work is one bit test of the node's unscheduled-job
-// mask and four integer ops.  Heads, tails and the unscheduled mask live in
-// VGPRs; the processing-time matrix is staged in LDS for the per-lane gathers
-// of the head/tail recurrences.  `groups` waves may split the pair range of
-// one 64-node tile (pairs-per-wavefront), combining through LDS.
+// (job, p_a, p_b, lag) is wave-uniform and arrives through scalar loads into
+// SGPRs; per lane a step is one bit extract from the node's unscheduled-job
+// mask, three 24-bit multiply-adds and a max.  The pair/step stream is
+// software-pipelined: the scalar loads of the next group of four steps are in
+// flight while the current group computes, across pair boundaries.  Heads,
+// tails and the job mask live in VGPRs (read with wave-uniform indices); the
+// processing-time matrix is staged in LDS for the per-lane gathers of the
+// head/tail recurrences.  `groups` waves may split the pair range of one
+// 64-node tile (pairs-per-wavefront), combining through LDS.
 //
 // Semantics follow bound_flowshop::bornes_calculer (src/bound_flowshop.cpp
 // 929-951): a leaf (limite2 - limite1 == 1) yields its makespan
 // (criteres_calculer 897-909); otherwise LB2 = max(0, max_s v_s) over the
 // pairs in order (borneInfMakespan 121-168), with v_s built from the head
 // start (initCmax 79-90), the Johnson scan over unscheduled jobs
-// (heuristiqueCmax 64-77) and the tail combine (cmaxFin 92-100 / 147-148).
+// (heuristiqueCmax 64-77) and the tail combine (cmaxFin 92-100, 147-148).
 #include "bound.h"
 
 namespace wsbb {
@@ -27,9 +30,27 @@ __device__ __forceinline__ void set_job_bit(uint32_t (&sm)[NW], int j) {
   for (int w = 0; w < NW; ++w) sm[w] |= ((j >> 5) == w) ? (1u << (j & 31)) : 0u;
 }
 
-template <int MM, int NW, class Src>
-__global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
-                                                    BoundArgs A) {
+__device__ __forceinline__ uint32_t bitsel(uint32_t mask, uint32_t a, uint32_t b) {
+  return (mask & a) | (~mask & b);  // v_bfi_b32
+}
+
+// four Johnson steps (one scalar-load group)
+template <int NW>
+struct StepGroup {
+  int4 rec[4];   // {job, p_a, p_b, kBias + lag + p_b}
+  uint2 sel[4];  // word-select masks of the job (NW > 1)
+  __device__ __forceinline__ void load(const DevTables& T, size_t step) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rec[u] = ld_const4(T.steps, step + u);
+    if constexpr (NW > 1) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) sel[u] = ld_const2u(T.wsel, step + u);
+    }
+  }
+};
+
+template <int MM, int NW, bool EXTRA, class Src>
+__global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundArgs A) {
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
   const int mn4 = (m * n + 3) & ~3;
@@ -51,6 +72,8 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
   const size_t ld = A.ld > 0 ? static_cast<size_t>(A.ld) : static_cast<size_t>(N);
   const int ntiles = (N + 63) >> 6;
   const int nbt = (ntiles + tpb - 1) / tpb;
+  const int inc = (A.inc_dev != nullptr) ? *A.inc_dev : 0;
+  const int gpp = T.n4 >> 2;  // step groups per pair
 
   uint32_t valid_bits[NW];
 #pragma unroll
@@ -60,7 +83,8 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
   }
 
   for (int bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
-    const int node = (bt * tpb + tib) * 64 + lane;
+    const int tile = __builtin_amdgcn_readfirstlane(bt * tpb + tib);
+    const int node = tile * 64 + lane;
     int l1 = -1, l2 = n, stride = 0, sw0 = -1, sw1 = -1;
     const uint8_t* col = nullptr;
     const bool valid = src.load(node, l1, l2, col, stride, sw0, sw1);
@@ -75,7 +99,7 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
     // (set_tempsMachinesFin_tempsJobFin 876-895); the reversed recurrence
     // below yields the same longest-path values for all machines at once.
     // q[k] = tail of machine k; q[k] for k >= m stays 0, which is the
-    // boundary of the reversed recurrence
+    // boundary of the reversed recurrence.
     int r[MM], q[MM];
     uint32_t sm[NW];
 #pragma unroll
@@ -107,7 +131,7 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
     // one-machine bound and selector features need the remaining work of
     // every machine: head (or R0) + sum of unscheduled p + tail (or Q0)
     int lb1 = 0;
-    if (A.kind == 1 || A.feat != nullptr) {
+    if (EXTRA) {
       int rem[MM];
 #pragma unroll
       for (int k = 0; k < MM; ++k) rem[k] = 0;
@@ -146,43 +170,73 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
 
     int lb = 0, fx = -1, am = -1;
     const bool use_best = A.best != -1;
-    for (int s = s_lo; s < s_hi; ++s) {
-      bool on = true;
-      if (A.sel != nullptr) {
-        on = valid && ((A.sel[static_cast<size_t>(s >> 5) * ld + node] >> (s & 31)) & 1u);
-        if (!__any(on)) continue;
+    const uint32_t* tsel =
+        (A.tile_sel != nullptr) ? A.tile_sel + static_cast<size_t>(tile) * A.nsw : nullptr;
+
+    // Pass 0 scans all pairs of [s_lo, s_hi), or the tile's learned subset;
+    // pass 1 (learned subset only) scans the complement for the nodes whose
+    // partial bound lands within `margin` below the incumbent (full LB2).
+    bool take = true;
+    long long fallbacks = 0;
+    int npairs = 0;  // pairs scanned by this wave
+    const int npass = (tsel != nullptr && A.margin >= 0) ? 2 : 1;
+    for (int ps = 0; ps < npass; ++ps) {
+      const uint32_t want = ps == 0 ? 1u : 0u;
+      if (ps == 1) {
+        take = valid && !leaf && lb < inc && lb >= inc - A.margin;
+        fallbacks = __popcll(__ballot(take));
+        if (fallbacks == 0) break;
       }
-      const int2 ab = ld_const2(T.pairs, s);
-      // pair start values: heads, or R0 without prefix (initCmax 79-90)
-      int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
-      int t1 = r[ab.y];
-      const int4* st = T.steps + static_cast<size_t>(s) * T.n4;
-      for (int k = 0; k < T.n4; k += 4) {
+      for (int s = s_lo; s < s_hi; ++s) {
+        if (tsel != nullptr) {
+          if (((ld_const_u32(tsel, s >> 5) >> (s & 31)) & 1u) != want) continue;
+        } else if (A.sel != nullptr) {
+          take = valid && ((A.sel[static_cast<size_t>(s >> 5) * ld + node] >> (s & 31)) & 1u);
+          if (!__any(take)) continue;
+        }
+        ++npairs;
+        const int2 ab = ld_const2(T.pairs, s);
+        int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
+        int t1 = r[ab.y];
+        const size_t base = static_cast<size_t>(s) * T.n4;
+        for (int k = 0; k < T.n4; k += 4) {
+          StepGroup<NW> g;
+          g.load(T, base + k);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int4 q = ld_const4(st, k + u);
-          const uint32_t word = (NW == 1) ? um[0] : um[q.x >> 5];
-          const uint32_t f = (word >> (q.x & 31)) & 1u;
-          // unscheduled job: t0 += p_a; t1 = max(t1, t0 + lag) + p_b.
-          // scheduled job (f = 0): both clocks unchanged, since the biased
-          // t0 stays below t1.
-          T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.y)) + T0);
-          const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.z)) + t1);
-          const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(q.w)) + T0);
-          t1 = max(x, y);
+          for (int u = 0; u < 4; ++u) {
+            const int4 qq = g.rec[u];
+            uint32_t word;
+            if constexpr (NW == 1) {
+              word = um[0];
+            } else if constexpr (NW == 2) {
+              word = bitsel(g.sel[u].x, um[1], um[0]);
+            } else {
+              const uint32_t w01 = bitsel(g.sel[u].x, um[1], um[0]);
+              const uint32_t w23 = bitsel(g.sel[u].x, um[3], um[2]);
+              word = bitsel(g.sel[u].y, w23, w01);
+            }
+            const uint32_t f = __builtin_amdgcn_ubfe(word, static_cast<uint32_t>(qq.x), 1u);
+            // unscheduled job: t0 += p_a; t1 = max(t1, t0 + lag) + p_b.
+            // scheduled job (f = 0): both clocks unchanged, since the biased
+            // t0 stays below t1.
+            T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.y)) + T0);
+            const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.z)) + t1);
+            const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.w)) + T0);
+            t1 = max(x, y);
+          }
         }
-      }
-      // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
-      // counts (borneInfMakespan 146-148)
-      const int qa = q[ab.x], qb = q[ab.y];
-      const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + ld_const(T.Q0, ab.y);
-      if (A.pairv != nullptr && valid) A.pairv[static_cast<size_t>(s) * ld + node] = v;
-      if (on) {
-        if (v > lb) {
-          lb = v;
-          am = s;
+        // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
+        // counts (borneInfMakespan 146-148)
+        const int qa = q[ab.x], qb = q[ab.y];
+        const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + ld_const(T.Q0, ab.y);
+        if (A.pairv != nullptr && valid) A.pairv[static_cast<size_t>(s) * ld + node] = v;
+        if (take) {
+          if (v > lb) {
+            lb = v;
+            am = s;
+          }
+          if (use_best && fx < 0 && lb > A.best) fx = lb;
         }
-        if (use_best && fx < 0 && lb > A.best) fx = lb;
       }
     }
 
@@ -214,10 +268,22 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src,
       A.out[node] = res;
       if (A.argmax != nullptr) A.argmax[node] = leaf ? -1 : am;
     }
+    if (A.stats != nullptr && tsel != nullptr && grp == 0) {
+      const int nvalid = __popcll(__ballot(valid));
+      if (lane == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.stats[0]),
+                  static_cast<unsigned long long>(nvalid));
+        if (fallbacks)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.stats[1]),
+                    static_cast<unsigned long long>(fallbacks));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.stats[2]),
+                  static_cast<unsigned long long>(nvalid) * static_cast<unsigned long long>(npairs));
+      }
+    }
   }
 }
 
-template <int MM, int NW, class Src>
+template <int MM, int NW, bool EXTRA, class Src>
 hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
                         int grid_cap, hipStream_t stream) {
   const int block = 256;
@@ -227,7 +293,7 @@ hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
   const size_t shmem = sizeof(int32_t) * (((T.m * T.n + 3) & ~3) + 4 * block);
-  hipLaunchKernelGGL((bound_kernel<MM, NW, Src>), dim3(grid), dim3(block), shmem,
+  hipLaunchKernelGGL((bound_kernel<MM, NW, EXTRA, Src>), dim3(grid), dim3(block), shmem,
                      stream, T, src, A);
   return hipGetLastError();
 }
@@ -236,11 +302,16 @@ template <class Src>
 hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
                       int grid_cap, hipStream_t stream) {
   const int nw = T.n <= 32 ? 1 : (T.n <= 64 ? 2 : 4);
-  const int mm = T.m <= 8 ? 8 : (T.m <= 16 ? 16 : 24);
-#define WSBB_DISPATCH(MMv, NWv) \
-  if (mm == MMv && nw == NWv) return launch_mm_nw<MMv, NWv, Src>(T, src, A, grid_cap, stream);
+  const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
+  const bool extra = A.kind == 1 || A.feat != nullptr;
+#define WSBB_DISPATCH(MMv, NWv)                                                   \
+  if (mm == MMv && nw == NWv)                                                     \
+    return extra ? launch_mm_nw<MMv, NWv, true, Src>(T, src, A, grid_cap, stream) \
+                 : launch_mm_nw<MMv, NWv, false, Src>(T, src, A, grid_cap, stream);
   WSBB_DISPATCH(8, 1) WSBB_DISPATCH(8, 2) WSBB_DISPATCH(8, 4)
+  WSBB_DISPATCH(12, 1) WSBB_DISPATCH(12, 2) WSBB_DISPATCH(12, 4)
   WSBB_DISPATCH(16, 1) WSBB_DISPATCH(16, 2) WSBB_DISPATCH(16, 4)
+  WSBB_DISPATCH(20, 1) WSBB_DISPATCH(20, 2) WSBB_DISPATCH(20, 4)
   WSBB_DISPATCH(24, 1) WSBB_DISPATCH(24, 2) WSBB_DISPATCH(24, 4)
 #undef WSBB_DISPATCH
   return hipErrorInvalidValue;

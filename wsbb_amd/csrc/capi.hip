@@ -165,20 +165,27 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   if ((e = hipEventCreate(&ctx->ev_stop)) != hipSuccess) return fail(e, "hipEventCreate");
 
   // Pack every table into one upload.  steps[s][k] = {job, p_a, p_b,
-  // kBias + lag + p_b} in Johnson order; padding steps are all zero (no-ops).
+  // kBias + lag + p_b} in Johnson order, wsel[s][k] the masks that pick the
+  // job's 32-bit word of a node's job mask; padding steps are all zero
+  // (no-ops), and 8 more follow the last pair for the scan's prefetch.
   const Instance& I = ctx->inst;
   const int P = I.P, n4 = (n + 3) & ~3;
-  std::vector<int4> steps(static_cast<size_t>(P) * n4 + 4, int4{0, 0, 0, 0});
+  const size_t nsteps = static_cast<size_t>(P) * n4 + 8;
+  std::vector<int4> steps(nsteps, int4{0, 0, 0, 0});
+  std::vector<uint2> wsel(nsteps, uint2{0u, 0u});
   for (int s = 0; s < P; ++s)
     for (int k = 0; k < n; ++k) {
       const int j = I.johnson[s * n + k];
       const int pa = I.p[I.pair_a[s] * n + j], pb = I.p[I.pair_b[s] * n + j];
-      steps[static_cast<size_t>(s) * n4 + k] = int4{j, pa, pb, kBias + I.lag[s * n + j] + pb};
+      const size_t at = static_cast<size_t>(s) * n4 + k;
+      steps[at] = int4{j, pa, pb, kBias + I.lag[s * n + j] + pb};
+      wsel[at] = uint2{((j >> 5) & 1) ? 0xffffffffu : 0u, ((j >> 6) & 1) ? 0xffffffffu : 0u};
     }
   std::vector<int2> pairs(P);
   for (int s = 0; s < P; ++s) pairs[s] = int2{I.pair_a[s], I.pair_b[s]};
   const size_t o_steps = 0;
-  const size_t o_pairs = o_steps + steps.size() * sizeof(int4);
+  const size_t o_wsel = o_steps + steps.size() * sizeof(int4);
+  const size_t o_pairs = o_wsel + wsel.size() * sizeof(uint2);
   const size_t o_p = (o_pairs + pairs.size() * sizeof(int2) + 255) & ~size_t(255);
   const size_t o_R0 = o_p + ((I.p.size() * 4 + 255) & ~size_t(255));
   const size_t o_Q0 = o_R0 + 256;
@@ -187,6 +194,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   if ((e = ctx->tables.reserve(total)) != hipSuccess) return fail(e, "hipMalloc(tables)");
   std::vector<char> blob(total, 0);
   std::memcpy(blob.data() + o_steps, steps.data(), steps.size() * sizeof(int4));
+  std::memcpy(blob.data() + o_wsel, wsel.data(), wsel.size() * sizeof(uint2));
   std::memcpy(blob.data() + o_pairs, pairs.data(), pairs.size() * sizeof(int2));
   std::memcpy(blob.data() + o_p, I.p.data(), I.p.size() * 4);
   std::memcpy(blob.data() + o_R0, I.R0.data(), m * 4);
@@ -200,6 +208,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   T.P = P;
   T.n4 = n4;
   T.steps = ctx->tables.as<const int4>(o_steps);
+  T.wsel = ctx->tables.as<const uint2>(o_wsel);
   T.pairs = ctx->tables.as<const int2>(o_pairs);
   T.p = ctx->tables.as<const int32_t>(o_p);
   T.R0 = ctx->tables.as<const int32_t>(o_R0);
@@ -469,6 +478,34 @@ int wsbb_autotune(wsbb_ctx* ctx, int N, int reps, int32_t* shapes, float* ms,
   while ((1 << (cls + 1)) <= tiles && cls < 31) ++cls;
   ctx->tuned_groups[cls] = best_g;
   if (ctx->launch.grid_cap == 0) ctx->launch.grid_cap = best_grid;
+  return WSBB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst,
+                                                const uint4* __restrict__ src, size_t n16) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n16;
+       i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int wsbb_copy_device(wsbb_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+  WSBB_CHECK_CTX(ctx);
+  if ((bytes & 15) || (reinterpret_cast<uintptr_t>(d_dst) & 15) ||
+      (reinterpret_cast<uintptr_t>(d_src) & 15)) {
+    set_error("copy needs 16-byte aligned pointers and sizes");
+    return WSBB_EINVAL;
+  }
+  const size_t n16 = bytes / 16;
+  const int grid = static_cast<int>(std::min<size_t>((n16 + 255) / 256, size_t(ctx->num_cus) * 16));
+  hipLaunchKernelGGL(k_copy16, dim3(grid > 0 ? grid : 1), dim3(256), 0, ctx->stream,
+                     static_cast<uint4*>(d_dst), static_cast<const uint4*>(d_src), n16);
+  WSBB_HIP(hipGetLastError(), "copy launch");
   return WSBB_OK;
 }
 

@@ -20,6 +20,13 @@ __device__ __forceinline__ int4 ld_const4(const int4* p, size_t i) {
   const int* q = reinterpret_cast<const int*>(p) + 4 * i;
   return make_int4(ld_const(q, 0), ld_const(q, 1), ld_const(q, 2), ld_const(q, 3));
 }
+__device__ __forceinline__ uint32_t ld_const_u32(const uint32_t* p, size_t i) {
+  return static_cast<uint32_t>(ld_const(reinterpret_cast<const int*>(p), i));
+}
+__device__ __forceinline__ uint2 ld_const2u(const uint2* p, size_t i) {
+  const int* q = reinterpret_cast<const int*>(p) + 2 * i;
+  return make_uint2(static_cast<uint32_t>(ld_const(q, 0)), static_cast<uint32_t>(ld_const(q, 1)));
+}
 
 // Instance tables resident in HBM (built once per instance on the host).
 struct DevTables {
@@ -27,7 +34,10 @@ struct DevTables {
   int n4;                // Johnson steps per pair, padded to a multiple of 4
   const int32_t* p;      // [m][n]
   const int2* pairs;     // [P] (machine a, machine b)
-  const int4* steps;     // [P][n4] {job, p_a, p_b, kBias + lag + p_b}; pad = 0
+  const int4* steps;     // [P][n4] {job, p_a, p_b, kBias + lag + p_b}; pad = 0,
+                         // plus 8 zero steps after the last pair
+  const uint2* wsel;     // [P][n4] masks selecting the job's 32-bit mask word:
+                         // x = all ones iff bit 0 of (job >> 5), y = bit 1
   const int32_t* R0;     // [m]
   const int32_t* Q0;     // [m]
   const int32_t* ptot;   // [m]
@@ -47,6 +57,14 @@ struct BoundArgs {
   const uint32_t* sel;   // [ceil(P/32)][N] pair subset per node, or null
   float* feat;           // [5m][N] selector features, or null
   float feat_scale;      // features are divided by this
+  const uint32_t* tile_sel;  // [ntiles][nsw] learned pair subset per 64-node
+                             // tile, or null
+  int nsw;                   // words per tile mask
+  int margin;                // with tile_sel: nodes whose subset bound is in
+                             // [inc - margin, inc) get full LB2 (-1: never)
+  const int* inc_dev;        // device incumbent for the margin test
+  long long* stats;          // with tile_sel: [0] nodes, [1] full-LB2
+                             // fallbacks, [2] node-pair evaluations
 };
 
 // Node source: explicit position-major nodes.

@@ -37,6 +37,7 @@
 #include "bound.h"
 #include "context.h"
 #include "search.h"
+#include "selector.h"
 
 namespace wsbb {
 
@@ -551,6 +552,43 @@ __global__ void k_init(SArgs S, int ub0) {
   }
 }
 
+// Records of the plain solver for the pair selector: every stride-th child
+// of the round (features and LB2-maximising pair) at a slot fixed by its
+// index, so the log does not depend on thread timing.
+struct LogArgs {
+  const Ctrl* ctrl;
+  const float* feat;  // [F][ld]
+  const int* label;   // [ld]
+  int ld, F, stride, cap;
+  float* log_feat;    // [F][cap]
+  int* log_label;     // [cap]
+  int* log_count;
+};
+
+__global__ __launch_bounds__(256) void k_log(LogArgs L) {
+  const int nc = L.ctrl->n_children;
+  const int base = *L.log_count;
+  const int nrec = (nc + L.stride - 1) / L.stride;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
+    const int slot = base + r;
+    if (slot >= L.cap) break;
+    const int c = r * L.stride;
+    for (int f = 0; f < L.F; ++f)
+      L.log_feat[static_cast<size_t>(f) * L.cap + slot] = L.feat[static_cast<size_t>(f) * L.ld + c];
+    L.log_label[slot] = L.label[c];
+  }
+}
+
+__global__ void k_log_advance(LogArgs L) {
+  const int nc = L.ctrl->n_children;
+  const int nrec = (nc + L.stride - 1) / L.stride;
+  *L.log_count = min(L.cap, *L.log_count + nrec);
+}
+
+__global__ void k_count_pairs(SArgs S) {
+  S.ctrl->stats[kStPairs] += static_cast<long long>(S.ctrl->n_children) * S.P;
+}
+
 }  // namespace
 
 struct SearchState {
@@ -558,15 +596,43 @@ struct SearchState {
   DevBuf mem;
   SArgs args{};
   Ctrl* host_ctrl = nullptr;  // pinned
+  // per-round selector buffers: features [5m][CCAP], labels, tile masks, LB1
+  DevBuf aux;
+  float* r_feat = nullptr;
+  int* r_label = nullptr;
+  uint32_t* r_tsel = nullptr;
+  int* r_lb1 = nullptr;
   // optional timing of every bound launch (pairs of events per round)
   std::vector<hipEvent_t> ev;
   double bound_ms = 0;
   long long bound_launches = 0;
 };
 
+static int ensure_aux(wsbb_ctx* ctx) {
+  SearchState* S = ctx->search;
+  if (S->r_feat) return WSBB_OK;
+  const int F = 5 * ctx->T.m, nsw = (ctx->T.P + 31) / 32;
+  const size_t ntiles = (static_cast<size_t>(S->CCAP) + 63) / 64;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t bf = al(sizeof(float) * F * static_cast<size_t>(S->CCAP));
+  const size_t bl = al(sizeof(int) * static_cast<size_t>(S->CCAP));
+  const size_t bt = al(sizeof(uint32_t) * ntiles * nsw);
+  hipError_t e = S->aux.reserve(bf + 2 * bl + bt);
+  if (e != hipSuccess) {
+    hip_fail(e, "hipMalloc(selector round buffers)");
+    return WSBB_ENOMEM;
+  }
+  S->r_feat = S->aux.as<float>(0);
+  S->r_label = S->aux.as<int>(bf);
+  S->r_lb1 = S->aux.as<int>(bf + bl);
+  S->r_tsel = S->aux.as<uint32_t>(bf + 2 * bl);
+  return WSBB_OK;
+}
+
 void destroy_search(wsbb_ctx* ctx) {
   if (!ctx->search) return;
   for (hipEvent_t e : ctx->search->ev) (void)hipEventDestroy(e);
+  ctx->search->aux.release();
   ctx->search->mem.release();
   if (ctx->search->host_ctrl) (void)hipHostFree(ctx->search->host_ctrl);
   delete ctx->search;
@@ -645,17 +711,60 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, st, A);
   hipLaunchKernelGGL(k_expand, dim3(pblocks), dim3(256), 0, st, A);
   ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
+  SearchState* SS = ctx->search;
+  const int log_stride = selector_log_stride(ctx);
   BoundArgs B{};
   B.N = A.CCAP;
   B.Ndev = &A.ctrl->n_children;
-  B.kind = kind;
+  B.kind = kind == WSBB_LB1 ? 1 : 0;
   B.best = -1;
   B.groups = ctx->groups_for(A.CCAP);
   B.out = A.c_bound;
+  B.feat_scale = static_cast<float>(ctx->inst.path_bound);
+  hipError_t e;
+  if (kind == WSBB_LEARNED) {
+    // features of every child, MLP pair subset per 64-child tile, then LB2
+    // over the subset with the full-LB2 fallback near the incumbent
+    BoundArgs Fa = B;
+    Fa.kind = 1;
+    Fa.groups = 1;
+    Fa.out = SS->r_lb1;
+    Fa.feat = SS->r_feat;
+    Fa.ld = A.CCAP;
+    if ((e = launch_bound(ctx->T, src, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+    if ((e = launch_mlp_select(ctx, SS->r_feat, A.CCAP, A.CCAP, &A.ctrl->n_children, SS->r_tsel,
+                               nullptr)) != hipSuccess)
+      return e;
+    B.groups = 1;
+    B.tile_sel = SS->r_tsel;
+    B.nsw = selector_nsw(ctx);
+    B.margin = selector_margin(ctx);
+    B.inc_dev = &A.ctrl->inc;
+    B.stats = &A.ctrl->stats[kStSubset];
+  } else if (log_stride > 0) {
+    B.feat = SS->r_feat;
+    B.argmax = SS->r_label;
+    B.ld = A.CCAP;
+  }
   if (ev0) (void)hipEventRecord(ev0, st);
-  hipError_t e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
+  e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
   if (e != hipSuccess) return e;
   if (ev1) (void)hipEventRecord(ev1, st);
+  if (kind != WSBB_LEARNED) hipLaunchKernelGGL(k_count_pairs, dim3(1), dim3(1), 0, st, A);
+  if (kind != WSBB_LEARNED && log_stride > 0) {
+    LogArgs L{};
+    L.ctrl = A.ctrl;
+    L.feat = SS->r_feat;
+    L.label = SS->r_label;
+    L.ld = A.CCAP;
+    L.F = 5 * ctx->T.m;
+    L.stride = log_stride;
+    selector_log_buffers(ctx, &L.log_feat, &L.log_label, &L.log_count, &L.cap);
+    if (L.cap > 0) {
+      hipLaunchKernelGGL(k_log, dim3(64), dim3(256), 0, st, L);
+      hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, L);
+    }
+  }
   hipLaunchKernelGGL(k_rank, dim3(rblocks < 4096 ? rblocks : 4096), dim3(256), 0, st, A);
   hipLaunchKernelGGL(k_incumbent, dim3(1), dim3(1024), 0, st, A);
   hipLaunchKernelGGL(k_keep, dim3(pblocks), dim3(256), 0, st, A);
@@ -673,8 +782,12 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     set_error("pool_size must exceed 2n+1 and pool_size*max_batches must be <= 2^23");
     return WSBB_EINVAL;
   }
-  if (kind != WSBB_LB2 && kind != WSBB_LB1) {
-    set_error("search kind must be WSBB_LB2 or WSBB_LB1");
+  if (kind != WSBB_LB2 && kind != WSBB_LB1 && kind != WSBB_LEARNED) {
+    set_error("search kind must be WSBB_LB2, WSBB_LB1 or WSBB_LEARNED");
+    return WSBB_EINVAL;
+  }
+  if (kind == WSBB_LEARNED && !selector_ready(ctx)) {
+    set_error("learned search needs a pair selector (wsbb_selector_init / train)");
     return WSBB_EINVAL;
   }
   hipError_t e = hipSetDevice(ctx->device);
@@ -682,6 +795,10 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   int rc = ensure_search(ctx, pool_size, max_batches, cap_nodes);
   if (rc) return rc;
   SearchState* S = ctx->search;
+  if (kind == WSBB_LEARNED || selector_log_stride(ctx) > 0) {
+    rc = ensure_aux(ctx);
+    if (rc) return rc;
+  }
   SArgs A = S->args;
   A.pool_m1 = pool_size - 1;
   A.max_batches = max_batches;

@@ -1,6 +1,829 @@
-// Learned machine-pair selector (placeholder until the selector lands).
+// Learned machine-pair selector: a two-layer MLP (5m features -> 128 hidden,
+// ReLU -> P pair scores) with BF16 weights and FP32 accumulation.
+//
+// Inference (the search's hot path) runs on MFMA.  It computes the transposed
+// product: weights are the A operand and node features the B operand of
+// v_mfma_f32_32x32x8_bf16, so the layer-1 accumulator of a lane holds exactly
+// the hidden units that lane must supply as layer-2 B operand; the two layers
+// chain in registers.  One wavefront handles one 64-node tile, turns the
+// scores into per-node softmax probabilities, sums them over the tile and
+// keeps the top-k pairs as the tile's pair subset (a wave-uniform subset keeps
+// the Johnson scan on the scalar path).  The max over any subset of pairs is
+// still a lower bound.
+//
+// Training is plain FP32 VALU work on small matrices: forward, softmax cross
+// entropy against the LB2-maximising pair logged by the plain solver,
+// backward, Adam on FP32 master weights; forward passes use the BF16-rounded
+// weights and activations the inference kernel sees.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/wsbb_amd.h"
 #include "context.h"
+#include "selector.h"
 
 namespace wsbb {
-void destroy_selector(wsbb_ctx* ctx) { (void)ctx; }
+
+namespace {
+
+constexpr int kHidden = 128;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  uint32_t u = static_cast<uint32_t>(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+__host__ __device__ inline float bfround(float f) { return bf2f(f2bf(f)); }
+
+// ---------------------------------------------------------------- inference
+struct MlpArgs {
+  int N, F, Fsteps, P, nsw, topk;
+  const int* Ndev;          // device node count, or null
+  const float* feat;        // [F][ld]
+  int ld;
+  const uint2* frag1;       // [4][Fsteps][64] A fragments of W1^T
+  const uint2* frag2;       // [PB][16][64]   A fragments of W2^T
+  const float* b1;          // [128]
+  const float* b2;          // [PB*32], padding = -inf
+  uint32_t* tile_sel;       // [ntiles][nsw]
+  float* scores;            // [P][ld] logits, or null
+};
+
+template <int PB>
+__global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
+  __shared__ float s_p[PB * 32][33];  // probabilities of one 32-node block
+  const int lane = threadIdx.x;
+  const int N = A.Ndev ? *A.Ndev : A.N;
+  const int ntiles = (N + 63) >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    float agg[PB / 2];  // pair sums, pairs lane + 64*i
+#pragma unroll
+    for (int i = 0; i < PB / 2; ++i) agg[i] = 0.f;
+    for (int nb = 0; nb < 2; ++nb) {
+      const int node = tile * 64 + nb * 32 + col;
+      const bool valid = node < N;
+      // layer 1: H^T[128 x 32] = W1^T[128 x Fp] . X^T[Fp x 32]
+      f32x16 acc1[4];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc1[bi][v] = 0.f;
+      for (int t = 0; t < A.Fsteps; ++t) {
+        bf16x4 b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 8 * t + 4 * half + i;
+          const float x = (valid && f < A.F) ? A.feat[static_cast<size_t>(f) * A.ld + node] : 0.f;
+          b[i] = static_cast<short>(f2bf(x));
+        }
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const uint2 a2 = A.frag1[(bi * A.Fsteps + t) * 64 + lane];
+          bf16x4 a;
+          a[0] = static_cast<short>(a2.x & 0xffff);
+          a[1] = static_cast<short>(a2.x >> 16);
+          a[2] = static_cast<short>(a2.y & 0xffff);
+          a[3] = static_cast<short>(a2.y >> 16);
+          acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc1[bi], 0, 0, 0);
+        }
+      }
+      // bias, ReLU, BF16: lane holds hidden 32bi + 8q + 4half + r for its node,
+      // which is the layer-2 B operand of k-step 4bi + q
+      bf16x4 hb[16];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int h = 32 * bi + 8 * q + 4 * half + r;
+            const float y = fmaxf(acc1[bi][4 * q + r] + A.b1[h], 0.f);
+            hb[4 * bi + q][r] = static_cast<short>(f2bf(y));
+          }
+      // layer 2: Z^T[PB*32 x 32] = W2^T . H^T
+      f32x16 acc2[PB];
+#pragma unroll
+      for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc2[bp][v] = 0.f;
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int bp = 0; bp < PB; ++bp) {
+          const uint2 a2 = A.frag2[(bp * 16 + t) * 64 + lane];
+          bf16x4 a;
+          a[0] = static_cast<short>(a2.x & 0xffff);
+          a[1] = static_cast<short>(a2.x >> 16);
+          a[2] = static_cast<short>(a2.y & 0xffff);
+          a[3] = static_cast<short>(a2.y >> 16);
+          acc2[bp] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[t], acc2[bp], 0, 0, 0);
+        }
+      // logits; this lane's pairs are 32bp + 8(v/4) + 4half + v%4
+      float mx = -INFINITY;
+#pragma unroll
+      for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int s = 32 * bp + 8 * (v >> 2) + 4 * half + (v & 3);
+          const float z = acc2[bp][v] + A.b2[s];
+          acc2[bp][v] = z;
+          if (A.scores != nullptr && valid && s < A.P) A.scores[static_cast<size_t>(s) * A.ld + node] = z;
+          mx = fmaxf(mx, z);
+        }
+      mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+      float sum = 0.f;
+#pragma unroll
+      for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const float e = __expf(acc2[bp][v] - mx);
+          acc2[bp][v] = e;
+          sum += e;
+        }
+      sum += __shfl_xor(sum, 32, 64);
+      const float inv = valid ? 1.f / sum : 0.f;
+#pragma unroll
+      for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int s = 32 * bp + 8 * (v >> 2) + 4 * half + (v & 3);
+          s_p[s][col] = acc2[bp][v] * inv;
+        }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PB / 2; ++i) {
+        const int s = lane + 64 * i;
+        float a = 0.f;
+        for (int c = 0; c < 32; ++c) a += s_p[s][c];
+        agg[i] += a;
+      }
+      __syncthreads();
+    }
+    // top-k pairs of the tile (ties: lower pair index), padding excluded
+    uint32_t mine[PB / 2];
+#pragma unroll
+    for (int i = 0; i < PB / 2; ++i) {
+      mine[i] = 0;
+      if (lane + 64 * i >= A.P) agg[i] = -1.f;
+    }
+    const int k = min(A.topk, A.P);
+    for (int it = 0; it < k; ++it) {
+      float bv = -2.f;
+      int bs = 1 << 30;
+#pragma unroll
+      for (int i = 0; i < PB / 2; ++i) {
+        const int s = lane + 64 * i;
+        if (!mine[i] && (agg[i] > bv || (agg[i] == bv && s < bs))) {
+          bv = agg[i];
+          bs = s;
+        }
+      }
+      for (int d = 32; d >= 1; d >>= 1) {
+        const float ov = __shfl_xor(bv, d, 64);
+        const int os = __shfl_xor(bs, d, 64);
+        if (ov > bv || (ov == bv && os < bs)) {
+          bv = ov;
+          bs = os;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PB / 2; ++i)
+        if (lane + 64 * i == bs) mine[i] = 1;
+    }
+    // pack the mask: pair lane + 64i sits in word 2i (lanes 0-31) or 2i+1
+    // (lanes 32-63), bit lane & 31
+#pragma unroll
+    for (int i = 0; i < PB / 2; ++i) {
+      const unsigned long long bal = __ballot(mine[i] != 0);
+      if (lane == 0) {
+        if (2 * i < A.nsw)
+          A.tile_sel[static_cast<size_t>(tile) * A.nsw + 2 * i] = static_cast<uint32_t>(bal);
+        if (2 * i + 1 < A.nsw)
+          A.tile_sel[static_cast<size_t>(tile) * A.nsw + 2 * i + 1] = static_cast<uint32_t>(bal >> 32);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- training
+struct TrainArgs {
+  int B, F, P;             // minibatch rows, features, pairs
+  const float* feat;       // [F][ldf] records
+  int ldf;
+  const int* label;        // [records]
+  const int* idx;          // [B] record index of each row
+  const float* W1;         // [F][128] masters
+  const float* b1;
+  const float* W2;         // [128][P]
+  const float* b2;
+  float* X;                // [B][F] bf16-rounded inputs
+  float* A1;               // [B][128] pre-activations
+  float* Hb;               // [B][128] bf16-rounded activations
+  float* Z;                // [B][P]
+  float* dZ;               // [B][P]
+  float* dH;               // [B][128]
+  float* loss;             // [B]
+  float* gW1;
+  float* gb1;
+  float* gW2;
+  float* gb2;
+  int nvalid;              // rows with a label (>= 0)
+};
+
+__global__ void k_tr_in(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.B * T.F) return;
+  const int b = i / T.F, f = i - b * T.F;
+  T.X[i] = bfround(T.feat[static_cast<size_t>(f) * T.ldf + T.idx[b]]);
+}
+
+__global__ void k_tr_hidden(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.B * kHidden) return;
+  const int b = i / kHidden, h = i - b * kHidden;
+  float a = 0.f;
+  for (int f = 0; f < T.F; ++f) a += T.X[b * T.F + f] * bfround(T.W1[f * kHidden + h]);
+  a += T.b1[h];
+  T.A1[i] = a;
+  T.Hb[i] = bfround(fmaxf(a, 0.f));
+}
+
+__global__ void k_tr_out(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.B * T.P) return;
+  const int b = i / T.P, s = i - b * T.P;
+  float z = 0.f;
+  for (int h = 0; h < kHidden; ++h) z += T.Hb[b * kHidden + h] * bfround(T.W2[h * T.P + s]);
+  T.Z[i] = z + T.b2[s];
+}
+
+__global__ void k_tr_softmax(TrainArgs T) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= T.B) return;
+  const int y = T.label[T.idx[b]];
+  const float* z = T.Z + static_cast<size_t>(b) * T.P;
+  float* dz = T.dZ + static_cast<size_t>(b) * T.P;
+  if (y < 0) {
+    for (int s = 0; s < T.P; ++s) dz[s] = 0.f;
+    T.loss[b] = 0.f;
+    return;
+  }
+  float mx = -INFINITY;
+  for (int s = 0; s < T.P; ++s) mx = fmaxf(mx, z[s]);
+  float sum = 0.f;
+  for (int s = 0; s < T.P; ++s) sum += expf(z[s] - mx);
+  const float inv = 1.f / sum, scale = 1.f / static_cast<float>(T.nvalid);
+  for (int s = 0; s < T.P; ++s) dz[s] = (expf(z[s] - mx) * inv - (s == y ? 1.f : 0.f)) * scale;
+  T.loss[b] = (logf(sum) + mx - z[y]) * scale;
+}
+
+__global__ void k_tr_gW2(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kHidden * T.P) {
+    const int h = i / T.P, s = i - h * T.P;
+    float g = 0.f;
+    for (int b = 0; b < T.B; ++b) g += T.Hb[b * kHidden + h] * T.dZ[static_cast<size_t>(b) * T.P + s];
+    T.gW2[i] = g;
+  } else if (i < kHidden * T.P + T.P) {
+    const int s = i - kHidden * T.P;
+    float g = 0.f;
+    for (int b = 0; b < T.B; ++b) g += T.dZ[static_cast<size_t>(b) * T.P + s];
+    T.gb2[s] = g;
+  }
+}
+
+__global__ void k_tr_dH(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.B * kHidden) return;
+  const int b = i / kHidden, h = i - b * kHidden;
+  float g = 0.f;
+  if (T.A1[i] > 0.f)
+    for (int s = 0; s < T.P; ++s) g += T.dZ[static_cast<size_t>(b) * T.P + s] * bfround(T.W2[h * T.P + s]);
+  T.dH[i] = g;
+}
+
+__global__ void k_tr_gW1(TrainArgs T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < T.F * kHidden) {
+    const int f = i / kHidden, h = i - f * kHidden;
+    float g = 0.f;
+    for (int b = 0; b < T.B; ++b) g += T.X[b * T.F + f] * T.dH[b * kHidden + h];
+    T.gW1[i] = g;
+  } else if (i < T.F * kHidden + kHidden) {
+    const int h = i - T.F * kHidden;
+    float g = 0.f;
+    for (int b = 0; b < T.B; ++b) g += T.dH[b * kHidden + h];
+    T.gb1[h] = g;
+  }
+}
+
+__global__ void k_adam(float* w, float* m, float* v, const float* g, int n, float lr,
+                       float b1c, float b2c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  const float mi = beta1 * m[i] + (1.f - beta1) * g[i];
+  const float vi = beta2 * v[i] + (1.f - beta2) * g[i] * g[i];
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / b1c) / (sqrtf(vi / b2c) + eps);
+}
+
+// inference fragments from the masters
+__global__ void k_frags(const float* W1, const float* W2, int F, int Fsteps, int P, int PB,
+                        uint2* frag1, uint2* frag2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = 4 * Fsteps * 64, n2 = PB * 16 * 64;
+  if (i < n1) {
+    const int lane = i & 63, t = (i >> 6) % Fsteps, bi = (i >> 6) / Fsteps;
+    const int h = 32 * bi + (lane & 31);
+    uint16_t v[4];
+    for (int r = 0; r < 4; ++r) {
+      const int f = 8 * t + 4 * (lane >> 5) + r;
+      v[r] = f < F ? f2bf(W1[f * kHidden + h]) : 0;
+    }
+    frag1[i] = make_uint2(v[0] | (static_cast<uint32_t>(v[1]) << 16),
+                          v[2] | (static_cast<uint32_t>(v[3]) << 16));
+  } else if (i < n1 + n2) {
+    const int j = i - n1;
+    const int lane = j & 63, t = (j >> 6) & 15, bp = j >> 10;
+    const int s = 32 * bp + (lane & 31);
+    uint16_t v[4];
+    for (int r = 0; r < 4; ++r) {
+      const int h = 8 * t + 4 * (lane >> 5) + r;
+      v[r] = s < P ? f2bf(W2[h * P + s]) : 0;
+    }
+    frag2[j] = make_uint2(v[0] | (static_cast<uint32_t>(v[1]) << 16),
+                          v[2] | (static_cast<uint32_t>(v[3]) << 16));
+  }
+}
+
+__global__ void k_pad_b2(const float* b2, float* b2p, int P, int Pp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < Pp) b2p[s] = s < P ? b2[s] : -INFINITY;
+}
+
+}  // namespace
+
+struct SelectorState {
+  int F = 0, Fsteps = 0, P = 0, PB = 0, nsw = 0;
+  int topk = 16, margin = 0;
+  // parameters: masters (W1, b1, W2, b2), Adam moments, gradients
+  DevBuf params, frags;
+  float *W1, *b1, *W2, *b2, *mW1, *mb1, *mW2, *mb2, *vW1, *vb1, *vW2, *vb2;
+  float *gW1, *gb1, *gW2, *gb2, *b2pad;
+  uint2 *frag1, *frag2;
+  bool frags_fresh = false;
+  long long adam_t = 0;
+  // records logged by the plain solver
+  DevBuf log;
+  int log_cap = 0, log_stride = 0;
+  float* log_feat = nullptr;   // [F][log_cap]
+  int* log_label = nullptr;    // [log_cap]
+  int* log_count = nullptr;    // device counter
+  // training scratch
+  DevBuf scratch;
+  int scratch_B = 0;
+  // inference scratch for host-buffer calls
+  DevBuf io;
+};
+
+void destroy_selector(wsbb_ctx* ctx) {
+  if (!ctx->selector) return;
+  SelectorState* S = ctx->selector;
+  S->params.release();
+  S->frags.release();
+  S->log.release();
+  S->scratch.release();
+  S->io.release();
+  delete S;
+  ctx->selector = nullptr;
+}
+
+static int param_count(const SelectorState& S) {
+  return S.F * kHidden + kHidden + kHidden * S.P + S.P;
+}
+
+static int ensure_selector(wsbb_ctx* ctx) {
+  if (ctx->selector) return WSBB_OK;
+  auto* S = new SelectorState();
+  ctx->selector = S;
+  S->F = 5 * ctx->T.m;
+  S->Fsteps = (S->F + 7) / 8;
+  S->P = ctx->T.P;
+  S->PB = (S->P + 31) / 32;
+  if (S->PB % 2) S->PB += 1;  // pairs are summed in lane + 64 i strips
+  S->nsw = (S->P + 31) / 32;
+  const int np = param_count(*S);
+  const size_t per = (static_cast<size_t>(np) * 4 + 255) & ~size_t(255);
+  hipError_t e = S->params.reserve(per * 4 + 4 * 32 * S->PB + 256);
+  if (e != hipSuccess) {
+    destroy_selector(ctx);
+    return hip_fail(e, "hipMalloc(selector)");
+  }
+  float* base[4];
+  for (int k = 0; k < 4; ++k) base[k] = S->params.as<float>(per * k);
+  auto split = [&](float* p, float** w1, float** bb1, float** w2, float** bb2) {
+    *w1 = p;
+    *bb1 = p + S->F * kHidden;
+    *w2 = *bb1 + kHidden;
+    *bb2 = *w2 + kHidden * S->P;
+  };
+  split(base[0], &S->W1, &S->b1, &S->W2, &S->b2);
+  split(base[1], &S->mW1, &S->mb1, &S->mW2, &S->mb2);
+  split(base[2], &S->vW1, &S->vb1, &S->vW2, &S->vb2);
+  split(base[3], &S->gW1, &S->gb1, &S->gW2, &S->gb2);
+  S->b2pad = S->params.as<float>(per * 4);
+  const size_t f1 = static_cast<size_t>(4) * S->Fsteps * 64, f2 = static_cast<size_t>(S->PB) * 16 * 64;
+  e = S->frags.reserve((f1 + f2) * sizeof(uint2));
+  if (e != hipSuccess) {
+    destroy_selector(ctx);
+    return hip_fail(e, "hipMalloc(selector frags)");
+  }
+  S->frag1 = S->frags.as<uint2>(0);
+  S->frag2 = S->frags.as<uint2>(f1 * sizeof(uint2));
+  return WSBB_OK;
+}
+
+static hipError_t refresh_frags(wsbb_ctx* ctx) {
+  SelectorState* S = ctx->selector;
+  if (S->frags_fresh) return hipSuccess;
+  const int n = 4 * S->Fsteps * 64 + S->PB * 16 * 64;
+  hipLaunchKernelGGL(k_frags, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, S->W1, S->W2,
+                     S->F, S->Fsteps, S->P, S->PB, S->frag1, S->frag2);
+  hipLaunchKernelGGL(k_pad_b2, dim3((32 * S->PB + 255) / 256), dim3(256), 0, ctx->stream, S->b2,
+                     S->b2pad, S->P, 32 * S->PB);
+  S->frags_fresh = true;
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_select(wsbb_ctx* ctx, const float* feat, int ld, int N, const int* Ndev,
+                             uint32_t* tile_sel, float* scores) {
+  SelectorState* S = ctx->selector;
+  hipError_t e = refresh_frags(ctx);
+  if (e != hipSuccess) return e;
+  MlpArgs A{};
+  A.N = N;
+  A.Ndev = Ndev;
+  A.F = S->F;
+  A.Fsteps = S->Fsteps;
+  A.P = S->P;
+  A.nsw = S->nsw;
+  A.topk = S->topk;
+  A.feat = feat;
+  A.ld = ld;
+  A.frag1 = S->frag1;
+  A.frag2 = S->frag2;
+  A.b1 = S->b1;
+  A.b2 = S->b2pad;
+  A.tile_sel = tile_sel;
+  A.scores = scores;
+  const int ntiles = (N + 63) / 64;
+  const int grid = std::max(1, std::min(ntiles, ctx->num_cus * 16));
+  switch (S->PB) {
+#define WSBB_PB(v) \
+  case v: hipLaunchKernelGGL(k_mlp_select<v>, dim3(grid), dim3(64), 0, ctx->stream, A); break;
+    WSBB_PB(2) WSBB_PB(4) WSBB_PB(6) WSBB_PB(8) WSBB_PB(10)
+#undef WSBB_PB
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int selector_topk(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->topk : 0; }
+int selector_margin(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->margin : 0; }
+bool selector_ready(const wsbb_ctx* ctx) { return ctx->selector != nullptr; }
+int selector_nsw(const wsbb_ctx* ctx) { return (ctx->T.P + 31) / 32; }
+
+// logging hooks used by the search
+int selector_log_stride(const wsbb_ctx* ctx) {
+  return ctx->selector ? ctx->selector->log_stride : 0;
+}
+void selector_log_buffers(const wsbb_ctx* ctx, float** feat, int** label, int** count,
+                          int* cap) {
+  const SelectorState* S = ctx->selector;
+  *feat = S->log_feat;
+  *label = S->log_label;
+  *count = S->log_count;
+  *cap = S->log_cap;
+}
+
+static hipError_t ensure_scratch(SelectorState* S, int B) {
+  if (S->scratch_B >= B) return hipSuccess;
+  const size_t bytes = sizeof(float) * (static_cast<size_t>(B) * (S->F + 3 * kHidden + 2 * S->P + 1)) +
+                       sizeof(int) * B + 1024;
+  hipError_t e = S->scratch.reserve(bytes);
+  if (e == hipSuccess) S->scratch_B = B;
+  return e;
+}
+
+static TrainArgs train_args(SelectorState* S, int B) {
+  TrainArgs T{};
+  T.B = B;
+  T.F = S->F;
+  T.P = S->P;
+  T.feat = S->log_feat;
+  T.ldf = S->log_cap;
+  T.label = S->log_label;
+  T.W1 = S->W1;
+  T.b1 = S->b1;
+  T.W2 = S->W2;
+  T.b2 = S->b2;
+  float* p = S->scratch.as<float>();
+  T.X = p;
+  p += static_cast<size_t>(B) * S->F;
+  T.A1 = p;
+  p += static_cast<size_t>(B) * kHidden;
+  T.Hb = p;
+  p += static_cast<size_t>(B) * kHidden;
+  T.Z = p;
+  p += static_cast<size_t>(B) * S->P;
+  T.dZ = p;
+  p += static_cast<size_t>(B) * S->P;
+  T.dH = p;
+  p += static_cast<size_t>(B) * kHidden;
+  T.loss = p;
+  p += B;
+  T.idx = reinterpret_cast<int*>(p);
+  T.gW1 = S->gW1;
+  T.gb1 = S->gb1;
+  T.gW2 = S->gW2;
+  T.gb2 = S->gb2;
+  return T;
+}
+
+static void launch_fwd_bwd(wsbb_ctx* ctx, const TrainArgs& T) {
+  hipStream_t st = ctx->stream;
+  auto g = [](long long n) { return dim3(static_cast<unsigned>((n + 255) / 256)); };
+  hipLaunchKernelGGL(k_tr_in, g(1LL * T.B * T.F), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_hidden, g(1LL * T.B * kHidden), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_out, g(1LL * T.B * T.P), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_softmax, g(T.B), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_gW2, g(1LL * kHidden * T.P + T.P), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_dH, g(1LL * T.B * kHidden), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_gW1, g(1LL * T.F * kHidden + kHidden), dim3(256), 0, st, T);
+}
+
 }  // namespace wsbb
+
+using namespace wsbb;
+
+#define SEL_CHECK(ctx)                         \
+  do {                                         \
+    if (!(ctx)) {                              \
+      set_error("null context");               \
+      return WSBB_EINVAL;                      \
+    }                                          \
+    int rc_ = ensure_selector(ctx);            \
+    if (rc_) return rc_;                       \
+  } while (0)
+
+#define SEL_HIP(call, what)                            \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, what);   \
+  } while (0)
+
+extern "C" {
+
+int wsbb_selector_init(wsbb_ctx* ctx, uint64_t seed) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  const int np = param_count(*S);
+  std::vector<float> h(np, 0.f);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  const float s1 = std::sqrt(2.f / S->F), s2 = std::sqrt(1.f / kHidden);
+  for (int i = 0; i < S->F * kHidden; ++i) h[i] = nd(rng) * s1;
+  float* w2 = h.data() + S->F * kHidden + kHidden;
+  for (int i = 0; i < kHidden * S->P; ++i) w2[i] = nd(rng) * s2;
+  SEL_HIP(hipMemcpy(S->W1, h.data(), sizeof(float) * np, hipMemcpyHostToDevice), "H2D params");
+  std::vector<float> z(np, 0.f);
+  SEL_HIP(hipMemcpy(S->mW1, z.data(), sizeof(float) * np, hipMemcpyHostToDevice), "H2D adam");
+  SEL_HIP(hipMemcpy(S->vW1, z.data(), sizeof(float) * np, hipMemcpyHostToDevice), "H2D adam");
+  S->adam_t = 0;
+  S->frags_fresh = false;
+  return WSBB_OK;
+}
+
+int wsbb_selector_params(wsbb_ctx* ctx, int set, float* W1, float* b1, float* W2, float* b2) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  struct {
+    float* host;
+    float* dev;
+    size_t n;
+  } parts[4] = {{W1, S->W1, size_t(S->F) * kHidden},
+                {b1, S->b1, size_t(kHidden)},
+                {W2, S->W2, size_t(kHidden) * S->P},
+                {b2, S->b2, size_t(S->P)}};
+  for (auto& pt : parts) {
+    if (!pt.host) continue;
+    if (set)
+      SEL_HIP(hipMemcpy(pt.dev, pt.host, pt.n * 4, hipMemcpyHostToDevice), "H2D params");
+    else
+      SEL_HIP(hipMemcpy(pt.host, pt.dev, pt.n * 4, hipMemcpyDeviceToHost), "D2H params");
+  }
+  if (set) S->frags_fresh = false;
+  return WSBB_OK;
+}
+
+int wsbb_selector_config(wsbb_ctx* ctx, int topk, int margin) {
+  SEL_CHECK(ctx);
+  if (topk < 1 || margin < 0) {
+    set_error("topk must be >= 1 and margin >= 0");
+    return WSBB_EINVAL;
+  }
+  ctx->selector->topk = topk;
+  ctx->selector->margin = margin;
+  return WSBB_OK;
+}
+
+int wsbb_selector_log(wsbb_ctx* ctx, int stride, int capacity) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (stride < 0 || capacity < 0) return WSBB_EINVAL;
+  S->log_stride = stride;
+  if (capacity != S->log_cap) {
+    S->log.release();
+    S->log_cap = 0;
+    if (capacity > 0) {
+      const size_t bytes = sizeof(float) * static_cast<size_t>(S->F) * capacity +
+                           sizeof(int) * static_cast<size_t>(capacity) + 256;
+      hipError_t e = S->log.reserve(bytes);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc(log)");
+      S->log_cap = capacity;
+      S->log_feat = S->log.as<float>();
+      S->log_label = reinterpret_cast<int*>(S->log_feat + static_cast<size_t>(S->F) * capacity);
+      S->log_count = S->log_label + capacity;
+    }
+  }
+  if (S->log_cap > 0) SEL_HIP(hipMemsetAsync(S->log_count, 0, sizeof(int), ctx->stream), "memset");
+  SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  return WSBB_OK;
+}
+
+int wsbb_selector_records(wsbb_ctx* ctx, float* feat, int32_t* label, int* count) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  int c = 0;
+  if (S->log_cap > 0)
+    SEL_HIP(hipMemcpy(&c, S->log_count, sizeof(int), hipMemcpyDeviceToHost), "D2H count");
+  c = std::min(c, S->log_cap);
+  if (count) *count = c;
+  if (c > 0 && feat)
+    for (int f = 0; f < S->F; ++f)
+      SEL_HIP(hipMemcpy(feat + static_cast<size_t>(f) * c, S->log_feat + static_cast<size_t>(f) * S->log_cap,
+                        sizeof(float) * c, hipMemcpyDeviceToHost),
+              "D2H feat");
+  if (c > 0 && label)
+    SEL_HIP(hipMemcpy(label, S->log_label, sizeof(int) * c, hipMemcpyDeviceToHost), "D2H label");
+  return WSBB_OK;
+}
+
+int wsbb_selector_set_records(wsbb_ctx* ctx, const float* feat, const int32_t* label, int count) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (count < 0 || (count > 0 && (!feat || !label))) return WSBB_EINVAL;
+  int rc = wsbb_selector_log(ctx, S->log_stride, count);
+  if (rc) return rc;
+  for (int f = 0; f < S->F; ++f)
+    SEL_HIP(hipMemcpy(S->log_feat + static_cast<size_t>(f) * S->log_cap, feat + static_cast<size_t>(f) * count,
+                      sizeof(float) * count, hipMemcpyHostToDevice),
+            "H2D feat");
+  SEL_HIP(hipMemcpy(S->log_label, label, sizeof(int) * count, hipMemcpyHostToDevice), "H2D label");
+  SEL_HIP(hipMemcpy(S->log_count, &count, sizeof(int), hipMemcpyHostToDevice), "H2D count");
+  return WSBB_OK;
+}
+
+// one forward + backward over the records idx[0..B) (host indices into the
+// record store); gradients and mean loss are copied out
+int wsbb_selector_grad(wsbb_ctx* ctx, const int32_t* idx, int B, float* loss, float* gW1,
+                       float* gb1, float* gW2, float* gb2) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (B <= 0 || !idx || S->log_cap == 0) return WSBB_EINVAL;
+  int c = 0;
+  SEL_HIP(hipMemcpy(&c, S->log_count, sizeof(int), hipMemcpyDeviceToHost), "D2H count");
+  std::vector<int> lab(std::min(c, S->log_cap));
+  if (!lab.empty())
+    SEL_HIP(hipMemcpy(lab.data(), S->log_label, sizeof(int) * lab.size(), hipMemcpyDeviceToHost), "D2H");
+  int nvalid = 0;
+  for (int b = 0; b < B; ++b) {
+    if (idx[b] < 0 || idx[b] >= static_cast<int>(lab.size())) {
+      set_error("record index out of range");
+      return WSBB_EINVAL;
+    }
+    nvalid += lab[idx[b]] >= 0;
+  }
+  SEL_HIP(ensure_scratch(S, B), "hipMalloc(train scratch)");
+  TrainArgs T = train_args(S, B);
+  T.nvalid = std::max(nvalid, 1);
+  SEL_HIP(hipMemcpyAsync(const_cast<int*>(T.idx), idx, sizeof(int) * B, hipMemcpyHostToDevice, ctx->stream), "H2D idx");
+  launch_fwd_bwd(ctx, T);
+  SEL_HIP(hipGetLastError(), "train launch");
+  std::vector<float> l(B);
+  SEL_HIP(hipMemcpyAsync(l.data(), T.loss, sizeof(float) * B, hipMemcpyDeviceToHost, ctx->stream), "D2H loss");
+  if (gW1) SEL_HIP(hipMemcpyAsync(gW1, S->gW1, sizeof(float) * S->F * kHidden, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  if (gb1) SEL_HIP(hipMemcpyAsync(gb1, S->gb1, sizeof(float) * kHidden, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  if (gW2) SEL_HIP(hipMemcpyAsync(gW2, S->gW2, sizeof(float) * kHidden * S->P, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  if (gb2) SEL_HIP(hipMemcpyAsync(gb2, S->gb2, sizeof(float) * S->P, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  if (loss) {
+    double s = 0;
+    for (float x : l) s += x;
+    *loss = static_cast<float>(s);
+  }
+  return WSBB_OK;
+}
+
+int wsbb_selector_train(wsbb_ctx* ctx, int epochs, int batch, float lr, uint64_t seed,
+                        float* epoch_loss) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (epochs < 1 || batch < 1 || !(lr > 0.f)) return WSBB_EINVAL;
+  int c = 0;
+  if (S->log_cap > 0)
+    SEL_HIP(hipMemcpy(&c, S->log_count, sizeof(int), hipMemcpyDeviceToHost), "D2H count");
+  c = std::min(c, S->log_cap);
+  std::vector<int> lab(c);
+  if (c) SEL_HIP(hipMemcpy(lab.data(), S->log_label, sizeof(int) * c, hipMemcpyDeviceToHost), "D2H");
+  std::vector<int> rows;
+  for (int i = 0; i < c; ++i)
+    if (lab[i] >= 0) rows.push_back(i);
+  if (rows.empty()) {
+    set_error("no labelled records to train on");
+    return WSBB_EINVAL;
+  }
+  batch = std::min<int>(batch, static_cast<int>(rows.size()));
+  SEL_HIP(ensure_scratch(S, batch), "hipMalloc(train scratch)");
+  TrainArgs T = train_args(S, batch);
+  T.nvalid = batch;
+  std::mt19937_64 rng(seed);
+  const int np = param_count(*S);
+  std::vector<float> lbuf(batch);
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(rows.begin(), rows.end(), rng);
+    double tot = 0;
+    int nb = 0;
+    for (size_t start = 0; start + batch <= rows.size(); start += batch) {
+      SEL_HIP(hipMemcpyAsync(const_cast<int*>(T.idx), rows.data() + start, sizeof(int) * batch,
+                             hipMemcpyHostToDevice, ctx->stream),
+              "H2D idx");
+      launch_fwd_bwd(ctx, T);
+      ++S->adam_t;
+      const float b1c = 1.f - std::pow(0.9f, static_cast<float>(S->adam_t));
+      const float b2c = 1.f - std::pow(0.999f, static_cast<float>(S->adam_t));
+      hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, S->W1, S->mW1,
+                         S->vW1, S->gW1, np, lr, b1c, b2c);
+      SEL_HIP(hipGetLastError(), "train launch");
+      SEL_HIP(hipMemcpyAsync(lbuf.data(), T.loss, sizeof(float) * batch, hipMemcpyDeviceToHost, ctx->stream),
+              "D2H loss");
+      SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+      for (float x : lbuf) tot += x;
+      ++nb;
+    }
+    if (epoch_loss) epoch_loss[ep] = nb ? static_cast<float>(tot / nb) : 0.f;
+  }
+  S->frags_fresh = false;
+  return WSBB_OK;
+}
+
+int wsbb_selector_infer(wsbb_ctx* ctx, const float* feat, int N, float* scores, uint32_t* tile_sel) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (N <= 0 || !feat) return WSBB_EINVAL;
+  const int ntiles = (N + 63) / 64;
+  const size_t fb = sizeof(float) * static_cast<size_t>(S->F) * N;
+  const size_t sb = sizeof(float) * static_cast<size_t>(S->P) * N;
+  const size_t tb = sizeof(uint32_t) * static_cast<size_t>(ntiles) * S->nsw;
+  SEL_HIP(S->io.reserve(fb + sb + tb + 512), "hipMalloc(io)");
+  float* dfeat = S->io.as<float>();
+  float* dsc = S->io.as<float>((fb + 255) & ~size_t(255));
+  uint32_t* dts = S->io.as<uint32_t>(((fb + 255) & ~size_t(255)) + ((sb + 255) & ~size_t(255)));
+  SEL_HIP(hipMemcpyAsync(dfeat, feat, fb, hipMemcpyHostToDevice, ctx->stream), "H2D feat");
+  SEL_HIP(launch_mlp_select(ctx, dfeat, N, N, nullptr, dts, scores ? dsc : nullptr), "mlp launch");
+  if (scores) SEL_HIP(hipMemcpyAsync(scores, dsc, sb, hipMemcpyDeviceToHost, ctx->stream), "D2H scores");
+  if (tile_sel) SEL_HIP(hipMemcpyAsync(tile_sel, dts, tb, hipMemcpyDeviceToHost, ctx->stream), "D2H sel");
+  SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  return WSBB_OK;
+}
+
+}  // extern "C"
