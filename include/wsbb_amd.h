@@ -105,6 +105,14 @@ int wsbb_bound_batch_ex(wsbb_ctx *ctx, const uint8_t *permT,
                         const int32_t *l1, const int32_t *l2, int N, int kind,
                         int best, int32_t *out, int32_t *argmax,
                         int32_t *pairv, const uint32_t *sel, float *feat);
+/* The learned search's bound, host-buffer form: LB2 restricted, for every
+ * 64-node tile t, to the pairs set in tile_sel [t*ceil(P/32) ..]; nodes whose
+ * subset bound lies in [inc - margin, inc) (margin >= 0) get full LB2.
+ * stats [3] (may be NULL): nodes, full-LB2 fallbacks, node-pair
+ * evaluations.  No counterpart in the reference. */
+int wsbb_bound_tiles(wsbb_ctx *ctx, const uint8_t *permT, const int32_t *l1,
+                     const int32_t *l2, int N, const uint32_t *tile_sel,
+                     int margin, int32_t inc, int32_t *out, int64_t *stats);
 /* Device-resident form: all pointers in HBM, asynchronous on the context's
  * stream (call wsbb_synchronize before reading d_out). */
 int wsbb_bound_device(wsbb_ctx *ctx, const uint8_t *d_permT,

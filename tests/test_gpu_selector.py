@@ -1,0 +1,176 @@
+"""Learned pair selector: kernels against torch (CPU, fp32) and the oracle.
+
+- training gradients (hand-written forward/backward kernels) against torch
+  autograd of the same network with the same BF16 roundings;
+- MFMA inference logits against the torch forward, tile subsets against a
+  numpy top-k of the summed softmax;
+- the subset bound of every node against the oracle's LB2 over that subset,
+  with the full-LB2 fallback inside the margin;
+- learned solves prove the plain optimum; training beats random weights.
+Floating-point tolerances are written next to each comparison."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import wsbb_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def torch_grads(params, feat, label, idx):
+    X = bf(torch.from_numpy(feat[:, idx].T.copy()))
+    y = torch.from_numpy(label[idx].astype(np.int64))
+    W1 = bf(torch.from_numpy(params["W1"])).requires_grad_()
+    W2 = bf(torch.from_numpy(params["W2"])).requires_grad_()
+    b1 = torch.from_numpy(params["b1"]).clone().requires_grad_()
+    b2 = torch.from_numpy(params["b2"]).clone().requires_grad_()
+    h = torch.relu(X @ W1 + b1)
+    hb = h + (bf(h) - h).detach()  # rounding with identity gradient
+    z = hb @ W2 + b2
+    ok = y >= 0
+    loss = torch.nn.functional.cross_entropy(z[ok], y[ok])
+    loss.backward()
+    return loss.item(), dict(W1=W1.grad.numpy(), b1=b1.grad.numpy(), W2=W2.grad.numpy(),
+                             b2=b2.grad.numpy())
+
+
+def torch_logits(params, feat):
+    X = bf(torch.from_numpy(feat.T.copy()))
+    h = bf(torch.relu(X @ bf(torch.from_numpy(params["W1"])) + torch.from_numpy(params["b1"])))
+    return (h @ bf(torch.from_numpy(params["W2"])) + torch.from_numpy(params["b2"])).numpy()
+
+
+def random_records(eng, R, seed):
+    rng = np.random.default_rng(seed)
+    feat = rng.random((5 * eng.m, R), dtype=np.float32)
+    label = rng.integers(0, eng.P, size=R).astype(np.int32)
+    label[rng.random(R) < 0.1] = -1
+    return feat, label
+
+
+@pytest.mark.parametrize("tid", [1, 21])
+def test_gradients_match_torch(tid):
+    p = oracle.taillard(tid)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=3)
+        params = eng.selector_params()
+        feat, label = random_records(eng, 300, 5)
+        eng.selector_set_records(feat, label)
+        idx = np.random.default_rng(6).choice(300, 128, replace=False).astype(np.int32)
+        loss, g = eng.selector_grad(idx)
+    tloss, tg = torch_grads(params, feat, label, idx)
+    # fp32 sums in a different order can flip a BF16 rounding of a hidden
+    # activation: tolerances are 1e-3 relative on the loss and 2% of each
+    # gradient tensor's largest entry elementwise; a wrong index, sign or
+    # transposed operand gives O(1) errors.
+    assert abs(loss - tloss) <= 1e-3 * abs(tloss)
+    for k in ("W1", "b1", "W2", "b2"):
+        scale = np.abs(tg[k]).max()
+        assert np.allclose(g[k], tg[k], rtol=0, atol=2e-2 * scale), k
+
+
+@pytest.mark.parametrize("tid", [1, 11, 21, 51])
+def test_inference_matches_torch_and_tile_topk(tid):
+    p = oracle.taillard(tid)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=7)
+        params = eng.selector_params()
+        rng = np.random.default_rng(8)
+        params["b2"] = rng.normal(0, 0.5, eng.P).astype(np.float32)
+        eng.selector_params(params)
+        topk = min(16, eng.P)
+        eng.selector_config(topk=topk, margin=0)
+        N = 1000
+        feat = rng.random((5 * eng.m, N), dtype=np.float32)
+        scores, tsel = eng.selector_infer(feat)
+    ref = torch_logits(params, feat).T  # [P][N]
+    # MFMA accumulates in fp32 in its own order: 2e-3 absolute on logits
+    assert np.allclose(scores, ref, rtol=0, atol=2e-3)
+    e = np.exp(ref - ref.max(axis=0))
+    prob = e / e.sum(axis=0)
+    for t in range((N + 63) // 64):
+        agg = prob[:, 64 * t:64 * t + 64].sum(axis=1)
+        chosen = [s for s in range(eng.P) if (tsel[t, s // 32] >> (s % 32)) & 1]
+        assert len(chosen) == topk
+        kth = np.sort(agg)[::-1][topk - 1]
+        # every chosen pair is within rounding of the k-th largest sum
+        assert agg[chosen].min() >= kth - 1e-4
+        assert (agg[[s for s in range(eng.P) if s not in chosen]] <= kth + 1e-4).all()
+
+
+@pytest.mark.parametrize("tid", [11, 51])
+def test_tile_subset_bounds_match_oracle(tid):
+    p = oracle.taillard(tid)[3]
+    m, n = p.shape
+    o = oracle.Oracle(p)
+    N = 640
+    permT, l1, l2 = oracle.random_nodes(n, N, 11)
+    rng = np.random.default_rng(12)
+    with wsbb_amd.Engine(p) as eng:
+        P, nsw = eng.P, (eng.P + 31) // 32
+        selb = rng.random(((N + 63) // 64, P)) < 0.15
+        selb[:, 0] = True
+        tsel = np.zeros(((N + 63) // 64, nsw), np.uint32)
+        for s in range(P):
+            tsel[:, s // 32] |= selb[:, s].astype(np.uint32) << np.uint32(s % 32)
+        full = eng.bound(permT, l1, l2)
+        inc = int(np.median(full))
+        sub, st0 = eng.bound_tiles(permT, l1, l2, tsel, margin=-1)
+        mixed, st1 = eng.bound_tiles(permT, l1, l2, tsel, margin=60, inc=inc)
+    for k in range(N):
+        perm = permT[:, k].astype(np.int32)
+        ref = o.bound_subset(perm, l1[k], l2[k], selb[k // 64].astype(np.uint8))
+        assert sub[k] == ref
+        flagged = ref < inc and ref >= inc - 60
+        assert mixed[k] == (full[k] if flagged else ref)
+    assert (sub <= full).all()
+    assert st0["nodes"] == N and st0["fallbacks"] == 0
+    assert st1["fallbacks"] == int(sum(1 for k in range(N) if inc - 60 <= sub[k] < inc))
+
+
+def test_logging_training_and_learned_solves():
+    """20x10 ta011: records from a plain solve train the selector; learned
+    solves prove the same optimum, and the trained network needs fewer pair
+    evaluations than plain LB2 and hits the LB2 value more often than a
+    random network."""
+    p = oracle.taillard(11)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=1)
+        eng.selector_log(stride=4, capacity=100000)
+        plain = eng.solve(pool_size=4096, max_batches=10)
+        feat, label = eng.selector_records()
+        eng.selector_log(stride=0, capacity=0)  # stop logging, drop the store
+        eng.selector_set_records(feat, label)
+        assert plain["best"] == 1582 and feat.shape[1] > 1000
+        assert ((label >= -1) & (label < eng.P)).all()
+        valid = label >= 0
+        # held-out batch for hit rates
+        permT, l1, l2 = oracle.random_nodes(20, 4096, 21)
+        full = eng.bound(permT, l1, l2)
+
+        def hit_rate():
+            fe = eng.bound(permT, l1, l2, kind=wsbb_amd.LB1, feat=True)["feat"]
+            _, tsel = eng.selector_infer(fe)
+            sub, _ = eng.bound_tiles(permT, l1, l2, tsel, margin=-1)
+            return float((sub == full).mean())
+
+        eng.selector_config(topk=8, margin=30)
+        h_random = hit_rate()
+        losses = eng.selector_train(epochs=4, batch=512, lr=2e-3, seed=2)
+        assert losses[-1] < losses[0]
+        h_trained = hit_rate()
+        assert h_trained > h_random
+        learned = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+    assert learned["proven"] and learned["best"] == plain["best"]
+    st = learned["stats"]
+    assert st["subset_bounded"] == st["bounded"]
+    assert st["pairs_evaluated"] < plain["stats"]["pairs_evaluated"]
+    print(f"hit rate random {h_random:.3f} trained {h_trained:.3f}; "
+          f"pair evaluations learned {st['pairs_evaluated']} plain "
+          f"{plain['stats']['pairs_evaluated']}; nodes learned {st['bounded']} plain "
+          f"{plain['stats']['bounded']}; fallbacks {st['fallbacks']}")

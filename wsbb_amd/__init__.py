@@ -54,6 +54,8 @@ def _load():
         "wsbb_bound_batch_ex": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  _i32p, _i32p, _i32p, _u32p, _f32p], ctypes.c_int),
         "wsbb_bound_device": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+        "wsbb_bound_tiles": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, _u32p, ctypes.c_int,
+                              ctypes.c_int32, _i32p, _i64p], ctypes.c_int),
         "wsbb_malloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
         "wsbb_free": ([_vp, _vp], ctypes.c_int),
         "wsbb_h2d": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
@@ -203,6 +205,21 @@ class Engine:
         if feat:
             res["feat"] = ft
         return res
+
+    def bound_tiles(self, permT, l1, l2, tile_sel, margin=-1, inc=0):
+        """LB2 over a pair subset per 64-node tile (+ full-LB2 fallback)."""
+        permT = np.ascontiguousarray(permT, np.uint8)
+        l1 = np.ascontiguousarray(l1, np.int32)
+        l2 = np.ascontiguousarray(l2, np.int32)
+        tile_sel = np.ascontiguousarray(tile_sel, np.uint32)
+        N = l1.shape[0]
+        assert tile_sel.shape == ((N + 63) // 64, (self.P + 31) // 32)
+        out = np.zeros(N, np.int32)
+        st = np.zeros(3, np.int64)
+        _check(lib.wsbb_bound_tiles(self.h, _p(permT, _u8p), _p(l1), _p(l2), N,
+                                    _p(tile_sel, _u32p), margin, inc, _p(out), _p(st, _i64p)),
+               "bound_tiles")
+        return out, dict(nodes=int(st[0]), fallbacks=int(st[1]), pair_evals=int(st[2]))
 
     # ---- device memory ----
     def malloc(self, nbytes):

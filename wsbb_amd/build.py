@@ -44,7 +44,22 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_plugin_example()
     return LIB
+
+
+def build_plugin_example():
+    """tests/cpp/plugin_explorer: the reference's DFS loop over the C++
+    plugin face (include/wsbb_plugins.hpp), linked against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "plugin_explorer.cpp")
+    out = os.path.join(root, "tests", "cpp", "plugin_explorer")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
+           "-o", out, "-L", HERE, "-lwsbb_amd", "-Wl,-rpath,$ORIGIN/../../wsbb_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("plugin example build failed:\n" + r.stdout + r.stderr)
+    return out
 
 
 if __name__ == "__main__":

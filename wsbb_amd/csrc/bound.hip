@@ -87,7 +87,9 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
     const int node = tile * 64 + lane;
     int l1 = -1, l2 = n, stride = 0, sw0 = -1, sw1 = -1;
     const uint8_t* col = nullptr;
-    const bool valid = src.load(node, l1, l2, col, stride, sw0, sw1);
+    // the live count (device-side in a search round) bounds the last tile;
+    // a source's own limit is its capacity
+    const bool valid = node < N && src.load(node, l1, l2, col, stride, sw0, sw1);
     if (!valid) {
       l1 = -1;
       l2 = n;

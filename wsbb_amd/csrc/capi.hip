@@ -311,6 +311,54 @@ int wsbb_bound_batch_ex(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
   return WSBB_OK;
 }
 
+int wsbb_bound_tiles(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
+                     const int32_t* l2, int N, const uint32_t* tile_sel, int margin,
+                     int32_t inc, int32_t* out, int64_t* stats) {
+  WSBB_CHECK_CTX(ctx);
+  if (N <= 0 || !permT || !l1 || !l2 || !tile_sel || !out) {
+    set_error("bad tile-bound arguments");
+    return WSBB_EINVAL;
+  }
+  int rc = check_nodes_host(ctx, permT, l1, l2, N);
+  if (rc) return rc;
+  const int n = ctx->T.n, nsw = (ctx->T.P + 31) / 32, ntiles = (N + 63) / 64;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_perm = 0, o_l1 = al(size_t(n) * N), o_l2 = o_l1 + al(4 * size_t(N));
+  const size_t o_out = o_l2 + al(4 * size_t(N)), o_ts = o_out + al(4 * size_t(N));
+  const size_t o_inc = o_ts + al(4 * size_t(ntiles) * nsw), o_st = o_inc + 256;
+  const size_t total = o_st + 256;
+  WSBB_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  WSBB_HIP(ctx->batch.reserve(total), "hipMalloc(batch)");
+  hipStream_t st = ctx->stream;
+  long long zero[4] = {0, 0, 0, 0};
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_perm), permT, size_t(n) * N, hipMemcpyHostToDevice, st), "H2D");
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_l1), l1, 4 * size_t(N), hipMemcpyHostToDevice, st), "H2D");
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_l2), l2, 4 * size_t(N), hipMemcpyHostToDevice, st), "H2D");
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_ts), tile_sel, 4 * size_t(ntiles) * nsw, hipMemcpyHostToDevice, st), "H2D");
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_inc), &inc, 4, hipMemcpyHostToDevice, st), "H2D");
+  WSBB_HIP(hipMemcpyAsync(ctx->batch.as<char>(o_st), zero, sizeof(zero), hipMemcpyHostToDevice, st), "H2D");
+  BatchSource src{ctx->batch.as<uint8_t>(o_perm), ctx->batch.as<int32_t>(o_l1),
+                  ctx->batch.as<int32_t>(o_l2), N};
+  BoundArgs A{};
+  A.N = N;
+  A.kind = 0;
+  A.best = -1;
+  A.groups = 1;
+  A.out = ctx->batch.as<int32_t>(o_out);
+  A.tile_sel = ctx->batch.as<const uint32_t>(o_ts);
+  A.nsw = nsw;
+  A.margin = margin;
+  A.inc_dev = ctx->batch.as<const int>(o_inc);
+  A.stats = ctx->batch.as<long long>(o_st);
+  WSBB_HIP(launch_bound(ctx->T, src, A, ctx->grid_cap(), st), "bound kernel launch");
+  WSBB_HIP(hipMemcpyAsync(out, A.out, 4 * size_t(N), hipMemcpyDeviceToHost, st), "D2H");
+  WSBB_HIP(hipMemcpyAsync(zero, A.stats, sizeof(zero), hipMemcpyDeviceToHost, st), "D2H");
+  WSBB_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (stats)
+    for (int i = 0; i < 3; ++i) stats[i] = zero[i];
+  return WSBB_OK;
+}
+
 int wsbb_bound_batch(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
                      const int32_t* l2, int N, int kind, int best, int32_t* out) {
   return wsbb_bound_batch_ex(ctx, permT, l1, l2, N, kind, best, out, nullptr,

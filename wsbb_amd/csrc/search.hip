@@ -49,7 +49,7 @@ enum : int {
   kStImproved, kStMaxPool, kStRounds, kStSubset, kStFallback, kStPairs, kStWork
 };
 constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
-constexpr int kSelWin = 1024;
+constexpr int kSelWin = 4096;  // candidates per select window (4 per thread)
 
 struct Ctrl {
   int status;
@@ -139,14 +139,43 @@ struct OpMin {
   __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
 
-// ---- select: one block; wave 0 runs the greedy sub-batch packing ---------
-__global__ __launch_bounds__(256) void k_select(SArgs S) {
-  __shared__ int s_w[kSelWin];
-  __shared__ int s_c[kSelWin];
-  __shared__ int s_state[4];
-  __shared__ int s_tmp[256];
-  Ctrl* C = S.ctrl;
+// exclusive prefix sum of one value per thread over a 1024-thread block;
+// *total receives the sum (all threads)
+__device__ __forceinline__ int block_exscan1024(int v, int* s_w16, int* total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int incl = wave_incl_sum(v, lane);
+  if (lane == 63) s_w16[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    const int x = lane < 16 ? s_w16[lane] : 0;
+    const int xi = wave_incl_sum(x, lane);
+    if (lane < 16) s_w16[16 + lane] = xi - x;  // exclusive wave offsets
+    if (lane == 15) s_w16[32] = xi;
+  }
+  __syncthreads();
+  const int res = s_w16[16 + wave] + incl - v;
+  *total = s_w16[32];
+  __syncthreads();
+  return res;
+}
+
+// ---- select: one block ------------------------------------------------------
+// Candidates are taken from the top of the stack in windows of kSelWin.  The
+// inclusive prefix sums P of the reservations 2(n - depth) (0 for parents
+// whose bound is not below the incumbent) locate every sub-batch boundary by
+// binary search: a sub-batch that opened at candidate b (with carried
+// reservation c) closes in front of the first candidate i with
+// c + P[i] - P[b-1] >= pool_size - 1, which is the parent push_back1 turns
+// away and the next sub-batch starts with.
+__global__ __launch_bounds__(1024) void k_select(SArgs S) {
+  __shared__ int s_p[kSelWin];  // prefix sums of reservations
+  __shared__ int s_c[kSelWin];  // children generated per candidate
+  __shared__ int s_w16[40];
+  __shared__ int s_state[8];
+  __shared__ int s_tmp[1024];
+  Ctrl* C = S.ctrl;
+  const int tid = threadIdx.x;
+  constexpr int kPer = kSelWin / 1024;
   if (C->status != kRunning) {
     if (tid == 0) {
       C->n_parents = 0;
@@ -165,78 +194,96 @@ __global__ __launch_bounds__(256) void k_select(SArgs S) {
     return;
   }
   const int inc = C->inc;
-  // wave-0 state (identical in all its lanes)
-  int size1 = 0, nb = 0, np = 0, pruned = 0, stop = 0;
-  int consumed = 0;
+  // block-uniform carried state: reservation of the open sub-batch, closed
+  // sub-batches, parents taken, pruned candidates, candidates consumed
+  int carry = 0, nb = 0, np = 0, pruned = 0, consumed = 0;
   for (;;) {
     const int win = min(kSelWin, top - consumed);
     if (win <= 0) break;
-    for (int t = tid; t < win; t += blockDim.x) {
-      const int si = top - 1 - (consumed + t);
-      const int l1 = S.s_l1[si], l2 = S.s_l2[si];
-      const int depth = (l1 + 1) + (n - l2);
-      const int u = n - depth;
-      s_w[t] = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
-      s_c[t] = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
+    // coalesced loads: candidate i = t + 1024 k is the i-th from the top
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + 1024 * k;
+      int w = 0, c = 0;
+      if (i < win) {
+        const int si = top - 1 - (consumed + i);
+        const int l1 = S.s_l1[si], l2 = S.s_l2[si];
+        const int depth = (l1 + 1) + (n - l2);
+        const int u = n - depth;
+        w = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
+        c = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
+      }
+      s_p[i] = w;
+      s_c[i] = c;
     }
     __syncthreads();
-    if (wave == 0) {
-      int used = win;  // candidates of this window consumed
-      for (int base = 0; base < win && !stop; base += 64) {
-        const int t = base + lane;
-        const bool valid = t < win;
-        const int W = valid ? s_w[t] : 0;
-        const int incl = wave_incl_sum(W, lane);
-        int start = 0;
-        for (;;) {
-          const int excl_start = (start == 0) ? 0 : __shfl(incl, start - 1, 64);
-          const bool brk = valid && lane >= start && W > 0 && (size1 + incl - excl_start >= S.pool_m1);
-          const unsigned long long bm = __ballot(brk);
-          const int end = bm ? (__ffsll(static_cast<long long>(bm)) - 1) : 64;
-          const bool in = valid && lane >= start && lane < end;
-          const bool tk = in && W > 0;
-          const unsigned long long tm = __ballot(tk);
-          const int before = __popcll(tm & ((1ull << lane) - 1ull));
-          if (tk && np + before < S.PCAP) {
-            S.p_sel[np + before] = top - 1 - (consumed + t);
-            S.p_cnt[np + before] = s_c[t];
-          }
-          np += __popcll(tm);
-          pruned += __popcll(__ballot(in && W == 0));
-          const int last = (end == 64) ? 63 : (end > start ? end - 1 : start);
-          const int upto = (end > start) ? __shfl(incl, last, 64) : excl_start;
-          size1 += upto - excl_start;
-          if (end == 64) break;
-          // the sub-batch closes in front of lane `end` (that parent is
-          // re-inserted and opens the next sub-batch)
-          ++nb;
-          if (nb == S.max_batches) {
-            stop = 1;
-            used = base + end;
-            break;
-          }
-          size1 = 0;
-          start = end;
+    // blocked inclusive scan: thread t owns candidates [kPer t, kPer t + kPer)
+    int loc[kPer], tsum = 0;
+    for (int k = 0; k < kPer; ++k) {
+      tsum += s_p[kPer * tid + k];
+      loc[k] = tsum;
+    }
+    int total;
+    const int off = block_exscan1024(tsum, s_w16, &total);
+    for (int k = 0; k < kPer; ++k) s_p[kPer * tid + k] = off + loc[k];
+    __syncthreads();
+    if (tid == 0) {
+      // first candidate whose prefix reaches `thr`, searching [lo, win)
+      auto first_at_least = [&](int lo, int thr) {
+        int hi = win;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_p[mid] >= thr) hi = mid;
+          else lo = mid + 1;
         }
+        return lo;
+      };
+      int end = win, stop = 0, base = 0, cur = carry;  // P[b-1] of the open sub-batch
+      int i = first_at_least(0, S.pool_m1 - cur);
+      while (i < win) {
+        ++nb;
+        if (nb == S.max_batches) {
+          stop = 1;
+          end = i;
+          break;
+        }
+        base = i > 0 ? s_p[i - 1] : 0;
+        cur = 0;
+        i = first_at_least(i + 1, base + S.pool_m1);
       }
-      if (lane == 0) {
-        s_state[0] = used;
-        s_state[1] = stop;
-      }
+      const int lastp = win > 0 ? s_p[win - 1] : 0;
+      s_state[0] = end;
+      s_state[1] = stop;
+      s_state[2] = cur + (stop ? 0 : lastp - base);  // reservation carried on
     }
     __syncthreads();
-    consumed += s_state[0];
-    const int st = s_state[1];
+    const int end = s_state[0], stop = s_state[1];
+    carry = s_state[2];
+    // parents taken in [0, end): candidates whose prefix steps up
+    int mine = 0;
+    for (int k = 0; k < kPer; ++k) {
+      const int i = kPer * tid + k;
+      const int prev = i > 0 ? s_p[i - 1] : 0;
+      mine += (i < end && s_p[i] > prev);
+    }
+    int ntaken;
+    int slot = np + block_exscan1024(mine, s_w16, &ntaken);
+    for (int k = 0; k < kPer; ++k) {
+      const int i = kPer * tid + k;
+      const int prev = i > 0 ? s_p[i - 1] : 0;
+      if (i < end && s_p[i] > prev) {
+        if (slot < S.PCAP) {
+          S.p_sel[slot] = top - 1 - (consumed + i);
+          S.p_cnt[slot] = s_c[i];
+        }
+        ++slot;
+      }
+    }
+    np += ntaken;
+    pruned += end - ntaken;
+    consumed += end;
     __syncthreads();
-    if (st) break;
+    if (stop) break;
   }
-  if (wave == 0 && lane == 0) {
-    s_state[2] = np;
-    s_state[3] = pruned;
-  }
-  __syncthreads();
-  np = s_state[2];
-  pruned = s_state[3];
   if (np > S.PCAP) {
     if (tid == 0) C->status = kOverflow;
     return;
@@ -708,7 +755,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   const int pblocks = (A.PCAP + 255) / 256;
   const int cblocks = (A.CCAP + 255) / 256;
   const int rblocks = (A.PCAP + 3) / 4;
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
   hipLaunchKernelGGL(k_expand, dim3(pblocks), dim3(256), 0, st, A);
   ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
   SearchState* SS = ctx->search;
