@@ -31,6 +31,19 @@ PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CUs x 64 lanes
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic(workload):
+    """HBM bytes per bound-kernel dispatch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel
+    calibration), or None when absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    return rec["bytes_per_dispatch"] if rec else None
+
+
 def node_work(n, m, l1, l2):
     """algorithmic int ops per node: P(n + 4u + 4) + 2m d1 + m(m+1) d2"""
     P = m * (m - 1) // 2
@@ -174,7 +187,10 @@ def main():
         "rounds_per_solve": sv["stats"]["rounds"],
         "roofline": {"bound": "valu", "kernel": "bound_kernel (children of a round)",
                      "achieved": ach, "peak": PEAK_VALU_TOPS, "unit": "Tint-op/s",
-                     "frac": (ach / PEAK_VALU_TOPS) if ach else None, "traffic": None,
+                     "frac": (ach / PEAK_VALU_TOPS) if ach else None,
+                     "traffic": measured_traffic("solve"),
+                     "algorithmic_bytes": sv["stats"]["bounded"] / max(sv["stats"]["rounds"], 1)
+                     * (20 + 12),
                      "avg_launch_us": sv["bound_ms"] * 1e3 / max(sv["launches"], 1),
                      "launches": sv["launches"],
                      "share_of_solve": bound_s / sv["elapsed"]},
@@ -196,7 +212,9 @@ def main():
             "roofline": {"bound": "valu", "achieved": ach_b, "peak": PEAK_VALU_TOPS,
                          "unit": "Tint-op/s", "frac": ach_b / PEAK_VALU_TOPS,
                          "hbm_gbs": bt["bytes"] / (bt["ms"] / 1e3) / 1e9,
-                         "hbm_frac": bt["bytes"] / (bt["ms"] / 1e3) / 1e9 / PEAK_HBM_GBS},
+                         "hbm_frac": bt["bytes"] / (bt["ms"] / 1e3) / 1e9 / PEAK_HBM_GBS,
+                         "traffic": measured_traffic("batched"),
+                         "algorithmic_bytes": bt["bytes"]},
             "autotune": bt["tuned"],
         }
         if cpu_b:

@@ -135,7 +135,8 @@ int wsbb_timer_start(wsbb_ctx *ctx);
 int wsbb_timer_stop(wsbb_ctx *ctx, float *ms);
 
 /* ---- launch shape and autotuning --------------------------------------------
- * groups: waves that split the pairs of one 64-node tile (1, 2 or 4);
+ * groups: waves that split the pairs of one 64-node tile (1, 2, 4, 8 or
+ * 16; 8 and 16 apply to plain LB2 with at most 16 machines, else 4);
  * grid_blocks: cap on 256-thread blocks (0 = automatic).  wsbb_autotune
  * times every shape on a synthetic batch of N nodes and keeps the fastest
  * for batches of that size class; shapes/times are written if non-NULL. */
@@ -155,7 +156,7 @@ int wsbb_autotune(wsbb_ctx *ctx, int N, int reps, int32_t *shapes,
  * 2*(n-depth) stay below pool_size-1 (problems::push_back1, problems.cpp
  * 64-78).  ub0: initial incumbent (INT32_MAX = none).  budget: maximum
  * decomposed parents (<= 0: none).  kind: WSBB_LB2, WSBB_LB1 or
- * WSBB_LEARNED (needs wsbb_selector_load).  Outputs: best makespan,
+ * WSBB_LEARNED (needs wsbb_selector_init).  Outputs: best makespan,
  * best_perm [n] (may be NULL), stats [WSBB_NSTATS] (may be NULL).
  * Returns WSBB_EBUDGET (with the incumbent so far) when the budget ran out. */
 int wsbb_solve(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,

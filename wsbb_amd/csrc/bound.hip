@@ -8,9 +8,10 @@
 // software-pipelined: the scalar loads of the next group of four steps are in
 // flight while the current group computes, across pair boundaries.  Heads,
 // tails and the job mask live in VGPRs (read with wave-uniform indices); the
-// processing-time matrix is staged in LDS for the per-lane gathers of the
-// head/tail recurrences.  `groups` waves may split the pair range of one
-// 64-node tile (pairs-per-wavefront), combining through LDS.
+// processing-time matrix and each lane's permutation are staged in LDS for
+// the per-lane gathers of the head/tail recurrences.  `groups` waves may
+// split the pair range of one 64-node tile (pairs-per-wavefront), combining
+// through LDS.
 //
 // Semantics follow bound_flowshop::bornes_calculer (src/bound_flowshop.cpp
 // 929-951): a leaf (limite2 - limite1 == 1) yields its makespan
@@ -49,14 +50,53 @@ struct StepGroup {
   }
 };
 
-template <int MM, int NW, bool EXTRA, class Src>
-__global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundArgs A) {
+// four Johnson steps of one pair: unscheduled job: t0 += p_a;
+// t1 = max(t1, t0 + lag) + p_b.  A scheduled job (f = 0) leaves both clocks
+// unchanged, since the biased t0 stays below t1.
+template <int NW>
+__device__ __forceinline__ void step4(const StepGroup<NW>& g, const uint32_t (&um)[NW],
+                                      int& T0, int& t1) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int4 qq = g.rec[u];
+    uint32_t word;
+    if constexpr (NW == 1) {
+      word = um[0];
+    } else if constexpr (NW == 2) {
+      word = bitsel(g.sel[u].x, um[1], um[0]);
+    } else {
+      const uint32_t w01 = bitsel(g.sel[u].x, um[1], um[0]);
+      const uint32_t w23 = bitsel(g.sel[u].x, um[3], um[2]);
+      word = bitsel(g.sel[u].y, w23, w01);
+    }
+    const uint32_t f = __builtin_amdgcn_ubfe(word, static_cast<uint32_t>(qq.x), 1u);
+    T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.y)) + T0);
+    const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.z)) + t1);
+    const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.w)) + T0);
+    t1 = max(x, y);
+  }
+}
+
+// MODE 0: plain LB2 (optionally with the `best` early exit) and nothing
+// else, so none of the optional outputs holds scalar registers in the pair
+// loop; MODE 1: LB2 with the optional outputs, pair masks and tile subsets;
+// MODE 2: MODE 1 plus LB1 and the selector features.
+template <int MM, int NW, int MODE, int BLOCK, class Src>
+__global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, BoundArgs A) {
+  constexpr bool LEAN = MODE == 0, EXTRA = MODE == 2;
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
-  const int mn4 = (m * n + 3) & ~3;
-  int32_t* sp = lds;                                 // [m][n]
+  // rows m..MM-1 of the staged times are zero: the recurrences then run
+  // over all MM machines without per-machine guards, and the padded heads
+  // and tails never exceed the real ones
+  const int mn4 = (MM * n + 3) & ~3;
+  int32_t* sp = lds;                                 // [MM][n]
   int4* part = reinterpret_cast<int4*>(lds + mn4);   // [blockDim] partials
-  for (int i = threadIdx.x; i < m * n; i += blockDim.x) sp[i] = T.p[i];
+  // [n][64 * tiles per block] bytes: the permutation columns of the block's
+  // tiles, staged once per tile so the head/tail recurrences read LDS
+  // instead of waiting on one global load per position
+  uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
   __syncthreads();
 
   const int G = A.groups;
@@ -72,7 +112,8 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
   const size_t ld = A.ld > 0 ? static_cast<size_t>(A.ld) : static_cast<size_t>(N);
   const int ntiles = (N + 63) >> 6;
   const int nbt = (ntiles + tpb - 1) / tpb;
-  const int inc = (A.inc_dev != nullptr) ? *A.inc_dev : 0;
+  int inc = 0;
+  if constexpr (!LEAN) inc = (A.inc_dev != nullptr) ? *A.inc_dev : 0;
   const int gpp = T.n4 >> 2;  // step groups per pair
 
   uint32_t valid_bits[NW];
@@ -95,6 +136,23 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
       l2 = n;
     }
     const int d1 = l1 + 1, d2 = n - l2;
+    // the tile's first wave stages it (all loads of a chunk are issued
+    // before the first LDS store waits); the other waves of the tile read it
+    // after the barrier
+    const uint8_t* mine = pl + tib * 64 + lane;
+    const int bstride = 64 * tpb;
+    if (grp == 0 && valid) {
+      for (int i0 = 0; i0 < n; i0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) pl[(i0 + u) * bstride + tib * 64 + lane] = static_cast<uint8_t>(v[u]);
+      }
+    }
+    if (G > 1) __syncthreads();
 
     // heads of the prefix (set_tempsMachines_retardDebut 841-856) and tails
     // of the suffix.  The reference runs one forward pass per start machine
@@ -110,25 +168,25 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
     for (int w = 0; w < NW; ++w) sm[w] = 0;
 
     for (int i = 0; i < d1; ++i) {
-      const int j = perm_at(col, stride, sw0, sw1, i);
+      const int j = perm_at(mine, bstride, sw0, sw1, i);
       set_job_bit<NW>(sm, j);
       r[0] += sp[j];
 #pragma unroll
       for (int k = 1; k < MM; ++k)
-        if (k < m) r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
+        r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
     }
     for (int i = n - 1; i >= l2; --i) {
-      const int j = perm_at(col, stride, sw0, sw1, i);
+      const int j = perm_at(mine, bstride, sw0, sw1, i);
       set_job_bit<NW>(sm, j);
 #pragma unroll
       for (int k = MM - 1; k >= 0; --k)
-        if (k < m) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
+        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
     }
     const bool leaf = (l2 - l1 == 1);
     int leafv = 0;
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      if (k < m) leafv = max(leafv, r[k] + q[k]);
+      leafv = max(leafv, r[k] + q[k]);
 
     // one-machine bound and selector features need the remaining work of
     // every machine: head (or R0) + sum of unscheduled p + tail (or Q0)
@@ -139,10 +197,10 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
       for (int k = 0; k < MM; ++k) rem[k] = 0;
       const int rem_hi = valid ? l2 : d1;  // padding lanes read no permutation
       for (int i = d1; i < rem_hi; ++i) {
-        const int j = perm_at(col, stride, sw0, sw1, i);
+        const int j = perm_at(mine, bstride, sw0, sw1, i);
 #pragma unroll
         for (int k = 0; k < MM; ++k)
-          if (k < m) rem[k] += sp[k * n + j];
+          rem[k] += sp[k * n + j];
       }
 #pragma unroll
       for (int k = 0; k < MM; ++k)
@@ -168,12 +226,13 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
     // pair start values: heads, or R0 without prefix (initCmax 79-90)
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      if (k < m && d1 == 0) r[k] = ld_const(T.R0, k);
+      if (d1 == 0) r[k] = ld_const(T.R0, k);  // zero past machine m
 
     int lb = 0, fx = -1, am = -1;
     const bool use_best = A.best != -1;
-    const uint32_t* tsel =
-        (A.tile_sel != nullptr) ? A.tile_sel + static_cast<size_t>(tile) * A.nsw : nullptr;
+    const uint32_t* tsel = nullptr;
+    if constexpr (!LEAN)
+      if (A.tile_sel != nullptr) tsel = A.tile_sel + static_cast<size_t>(tile) * A.nsw;
 
     // Pass 0 scans all pairs of [s_lo, s_hi), or the tile's learned subset;
     // pass 1 (learned subset only) scans the complement for the nodes whose
@@ -181,6 +240,26 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
     bool take = true;
     long long fallbacks = 0;
     int npairs = 0;  // pairs scanned by this wave
+    if constexpr (LEAN) {
+      npairs = s_hi - s_lo;
+      for (int s = s_lo; s < s_hi; ++s) {
+        const int2 ab = ld_const2(T.pairs, s);
+        int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
+        int t1 = r[ab.y];
+        const size_t base = static_cast<size_t>(s) * T.n4;
+        for (int k = 0; k < T.n4; k += 4) {
+          StepGroup<NW> g;
+          g.load(T, base + k);
+          step4<NW>(g, um, T0, t1);
+        }
+        // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
+        // counts (borneInfMakespan 146-148)
+        const int qa = q[ab.x], qb = q[ab.y], q0b = ld_const(T.Q0, ab.y);
+        const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + q0b;
+        lb = max(lb, v);
+        if (use_best && fx < 0 && lb > A.best) fx = lb;
+      }
+    } else {
     const int npass = (tsel != nullptr && A.margin >= 0) ? 2 : 1;
     for (int ps = 0; ps < npass; ++ps) {
       const uint32_t want = ps == 0 ? 1u : 0u;
@@ -204,28 +283,7 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
         for (int k = 0; k < T.n4; k += 4) {
           StepGroup<NW> g;
           g.load(T, base + k);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int4 qq = g.rec[u];
-            uint32_t word;
-            if constexpr (NW == 1) {
-              word = um[0];
-            } else if constexpr (NW == 2) {
-              word = bitsel(g.sel[u].x, um[1], um[0]);
-            } else {
-              const uint32_t w01 = bitsel(g.sel[u].x, um[1], um[0]);
-              const uint32_t w23 = bitsel(g.sel[u].x, um[3], um[2]);
-              word = bitsel(g.sel[u].y, w23, w01);
-            }
-            const uint32_t f = __builtin_amdgcn_ubfe(word, static_cast<uint32_t>(qq.x), 1u);
-            // unscheduled job: t0 += p_a; t1 = max(t1, t0 + lag) + p_b.
-            // scheduled job (f = 0): both clocks unchanged, since the biased
-            // t0 stays below t1.
-            T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.y)) + T0);
-            const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.z)) + t1);
-            const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.w)) + T0);
-            t1 = max(x, y);
-          }
+          step4<NW>(g, um, T0, t1);
         }
         // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
         // counts (borneInfMakespan 146-148)
@@ -240,6 +298,8 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
           if (use_best && fx < 0 && lb > A.best) fx = lb;
         }
       }
+    }
+
     }
 
     if (G > 1) {
@@ -268,9 +328,10 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
       else if (A.kind == 1) res = lb1;
       else res = fx >= 0 ? fx : lb;
       A.out[node] = res;
-      if (A.argmax != nullptr) A.argmax[node] = leaf ? -1 : am;
+      if constexpr (!LEAN)
+        if (A.argmax != nullptr) A.argmax[node] = leaf ? -1 : am;
     }
-    if (A.stats != nullptr && tsel != nullptr && grp == 0) {
+    if (!LEAN && A.stats != nullptr && tsel != nullptr && grp == 0) {
       const int nvalid = __popcll(__ballot(valid));
       if (lane == 0) {
         atomicAdd(reinterpret_cast<unsigned long long*>(&A.stats[0]),
@@ -285,17 +346,18 @@ __global__ __launch_bounds__(256) void bound_kernel(DevTables T, Src src, BoundA
   }
 }
 
-template <int MM, int NW, bool EXTRA, class Src>
+template <int MM, int NW, int MODE, int BLOCK, class Src>
 hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
                         int grid_cap, hipStream_t stream) {
-  const int block = 256;
+  const int block = BLOCK;
   const int tpb = (block / 64) / A.groups;
   const int ntiles = (A.N + 63) / 64;
   int grid = (ntiles + tpb - 1) / tpb;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * (((T.m * T.n + 3) & ~3) + 4 * block);
-  hipLaunchKernelGGL((bound_kernel<MM, NW, EXTRA, Src>), dim3(grid), dim3(block), shmem,
+  const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + 4 * block) +
+                       static_cast<size_t>(T.n) * 64 * tpb;
+  hipLaunchKernelGGL((bound_kernel<MM, NW, MODE, BLOCK, Src>), dim3(grid), dim3(block), shmem,
                      stream, T, src, A);
   return hipGetLastError();
 }
@@ -305,11 +367,28 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
                       int grid_cap, hipStream_t stream) {
   const int nw = T.n <= 32 ? 1 : (T.n <= 64 ? 2 : 4);
   const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
-  const bool extra = A.kind == 1 || A.feat != nullptr;
-#define WSBB_DISPATCH(MMv, NWv)                                                   \
-  if (mm == MMv && nw == NWv)                                                     \
-    return extra ? launch_mm_nw<MMv, NWv, true, Src>(T, src, A, grid_cap, stream) \
-                 : launch_mm_nw<MMv, NWv, false, Src>(T, src, A, grid_cap, stream);
+  const int mode = (A.kind == 1 || A.feat != nullptr) ? 2
+                   : (A.kind == 0 && A.argmax == nullptr && A.pairv == nullptr &&
+                      A.sel == nullptr && A.tile_sel == nullptr && A.stats == nullptr)
+                       ? 0
+                       : 1;
+  // 8 or 16 waves per tile (1024-thread blocks) for small batches; the lean
+  // variants for up to 16 machines fit four such waves per SIMD
+  bool wide = A.groups > 4;
+  if (wide && (mode != 0 || mm > 16)) {
+    // the general and wide-machine variants stop at four waves per tile
+    BoundArgs A4 = A;
+    A4.groups = 4;
+    return launch_src(T, src, A4, grid_cap, stream);
+  }
+#define WSBB_DISPATCH(MMv, NWv)                                                         \
+  if (mm == MMv && nw == NWv) {                                                         \
+    if constexpr (MMv <= 16)                                                            \
+      if (wide) return launch_mm_nw<MMv, NWv, 0, 1024, Src>(T, src, A, grid_cap, stream); \
+    return mode == 0   ? launch_mm_nw<MMv, NWv, 0, 256, Src>(T, src, A, grid_cap, stream) \
+           : mode == 1 ? launch_mm_nw<MMv, NWv, 1, 256, Src>(T, src, A, grid_cap, stream) \
+                       : launch_mm_nw<MMv, NWv, 2, 256, Src>(T, src, A, grid_cap, stream); \
+  }
   WSBB_DISPATCH(8, 1) WSBB_DISPATCH(8, 2) WSBB_DISPATCH(8, 4)
   WSBB_DISPATCH(12, 1) WSBB_DISPATCH(12, 2) WSBB_DISPATCH(12, 4)
   WSBB_DISPATCH(16, 1) WSBB_DISPATCH(16, 2) WSBB_DISPATCH(16, 4)

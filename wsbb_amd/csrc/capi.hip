@@ -439,8 +439,8 @@ int wsbb_timer_stop(wsbb_ctx* ctx, float* ms) {
 
 int wsbb_set_launch(wsbb_ctx* ctx, int groups, int grid_blocks) {
   WSBB_CHECK_CTX(ctx);
-  if (groups != 0 && groups != 1 && groups != 2 && groups != 4) {
-    set_error("groups must be 1, 2 or 4 (0 = automatic)");
+  if (groups != 0 && groups != 1 && groups != 2 && groups != 4 && groups != 8 && groups != 16) {
+    set_error("groups must be 1, 2, 4, 8 or 16 (0 = automatic)");
     return WSBB_EINVAL;
   }
   if (grid_blocks < 0) return WSBB_EINVAL;
@@ -475,13 +475,16 @@ int wsbb_autotune(wsbb_ctx* ctx, int N, int reps, int32_t* shapes, float* ms,
   (void)hipMemcpy(dp, permT.data(), permT.size(), hipMemcpyHostToDevice);
   (void)hipMemcpy(d1, l1.data(), 4 * size_t(N), hipMemcpyHostToDevice);
   (void)hipMemcpy(d2, l2.data(), 4 * size_t(N), hipMemcpyHostToDevice);
-  const int cand_groups[3] = {1, 2, 4};
+  // 8 and 16 waves per tile run in 1024-thread blocks (up to 16 machines)
+  const int cand_groups[5] = {1, 2, 4, 8, 16};
+  const int n_groups = ctx->T.m <= 16 ? 5 : 3;
   const int cand_grid[3] = {ctx->num_cus * 4, ctx->num_cus * 8, ctx->num_cus * 16};
   int best_g = 1, best_grid = 0, idx = 0;
   float best_t = 1e30f;
   const Launch saved = ctx->launch;
-  for (int g : cand_groups)
+  for (int gi = 0; gi < n_groups; ++gi)
     for (int grid : cand_grid) {
+      const int g = cand_groups[gi];
       ctx->launch.groups = g;
       ctx->launch.grid_cap = grid;
       BatchSource src{static_cast<uint8_t*>(dp), static_cast<int32_t*>(d1),

@@ -20,11 +20,11 @@
 //   rank     per parent: branching side (choise, decompose_beginend.cpp
 //            39-50 with best = 99999), ranks by (bound, job) (problems::ranks,
 //            problems.cpp:46-62 with select_worst), leaf minimum
-//   incumb   incumbent seen by each parent in pop order: the running minimum
-//            of earlier parents' leaves (solutions::insert strictly-better
-//            rule, solutions.cpp:45-65; prune loop gpu.cu:317-394)
-//   keep     children of the chosen side with bound < that incumbent
-//   offsets  prefix sums of kept children, overflow check
+//   settle   incumbent seen by each parent in pop order: the running
+//            minimum of earlier parents' leaves (solutions::insert
+//            strictly-better rule, solutions.cpp:45-65; prune loop
+//            gpu.cu:317-394); children of the chosen side with bound below
+//            it are kept; prefix sums of kept children, overflow check
 //   push     write survivors onto the stack
 #include <hip/hip_runtime.h>
 
@@ -59,15 +59,15 @@ struct Ctrl {
   int n_children;
   int push_base;
   int push_total;
-  int best_child;  // child index whose leaf set the incumbent this round
   int have_perm;
-  int pad[7];
+  int pad[8];
   long long stats[WSBB_NSTATS];
   uint8_t best_perm[kMaxJobs];
 };
 
 struct SArgs {
   int n, m, P, CAP, PCAP, CCAP, pool_m1, max_batches;
+  int count_pairs;  // every child evaluates all pairs (plain LB2)
   long long budget;
   Ctrl* ctrl;
   uint8_t* s_perm;  // [n][CAP]
@@ -79,14 +79,14 @@ struct SArgs {
   int* p_l2;
   int* p_cost;
   int* p_sel;       // stack index of parent
-  int* p_cnt;       // children of parent
   int* p_off;       // [PCAP+1] first child
   int* p_side;
   int* p_nch;       // children on the chosen side
   int* p_leafmin;
   int* p_leafmax;   // cost of the leaf a simple parent inserts first
   int* p_leafwin;
-  int* p_incb;
+  int* p_kept0;     // chosen-side children with bound below the round's
+                    // starting incumbent
   int* p_kept;
   int* p_koff;
   int* c_desc;
@@ -95,43 +95,6 @@ struct SArgs {
   int* c_scost;
 };
 
-__device__ __forceinline__ int wave_incl_sum(int v, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-// exclusive block scan over in[0..L) with op/identity; returns the total
-template <class Op>
-__device__ int block_scan_excl(const int* in, int* out, int L, int ident, Op op,
-                               int* s_tmp) {
-  const int B = blockDim.x, t = threadIdx.x;
-  const int seg = (L + B - 1) / B;
-  const int lo = min(t * seg, L), hi = min(lo + seg, L);
-  int loc = ident;
-  for (int i = lo; i < hi; ++i) loc = op(loc, in[i]);
-  s_tmp[t] = loc;
-  __syncthreads();
-  for (int d = 1; d < B; d <<= 1) {
-    const int o = (t >= d) ? s_tmp[t - d] : ident;
-    __syncthreads();
-    s_tmp[t] = op(s_tmp[t], o);
-    __syncthreads();
-  }
-  int run = (t == 0) ? ident : s_tmp[t - 1];
-  const int total = s_tmp[B - 1];
-  for (int i = lo; i < hi; ++i) {
-    const int x = in[i];
-    out[i] = run;
-    run = op(run, x);
-  }
-  __syncthreads();
-  return total;
-}
-
 struct OpAdd {
   __device__ int operator()(int a, int b) const { return a + b; }
 };
@@ -139,42 +102,79 @@ struct OpMin {
   __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
 
-// exclusive prefix sum of one value per thread over a 1024-thread block;
-// *total receives the sum (all threads)
-__device__ __forceinline__ int block_exscan1024(int v, int* s_w16, int* total) {
+template <class Op>
+__device__ __forceinline__ int wave_incl_scan(int v, int lane, int ident, Op op) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d, 64);
+    v = op(v, lane >= d ? o : ident);
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread over a 1024-thread block; *total
+// receives the reduction of all values (every thread)
+template <class Op>
+__device__ __forceinline__ int block_exscan1024(int v, int ident, Op op, int* s_w16,
+                                                int* total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int incl = wave_incl_sum(v, lane);
+  const int incl = wave_incl_scan(v, lane, ident, op);
+  const int excl_w = __shfl_up(incl, 1, 64);
   if (lane == 63) s_w16[wave] = incl;
   __syncthreads();
   if (wave == 0) {
-    const int x = lane < 16 ? s_w16[lane] : 0;
-    const int xi = wave_incl_sum(x, lane);
-    if (lane < 16) s_w16[16 + lane] = xi - x;  // exclusive wave offsets
+    const int x = lane < 16 ? s_w16[lane] : ident;
+    const int xi = wave_incl_scan(x, lane, ident, op);
+    const int xe = __shfl_up(xi, 1, 64);
+    if (lane < 16) s_w16[16 + lane] = lane == 0 ? ident : xe;  // exclusive wave prefixes
     if (lane == 15) s_w16[32] = xi;
   }
   __syncthreads();
-  const int res = s_w16[16 + wave] + incl - v;
+  const int res = op(s_w16[16 + wave], lane == 0 ? ident : excl_w);
   *total = s_w16[32];
   __syncthreads();
   return res;
+}
+__device__ __forceinline__ int block_exscan1024(int v, int* s_w16, int* total) {
+  return block_exscan1024(v, 0, OpAdd(), s_w16, total);
+}
+
+// first index i in [lo, hi) with a[i] >= thr for nondecreasing a (hi if
+// none), by the whole wave: one probe per lane narrows to 64 candidates,
+// a second pass picks the first
+__device__ __forceinline__ int wave_first_at_least(const int* a, int lo, int hi, int thr,
+                                                   int lane) {
+  const int L = hi - lo;
+  if (L <= 0) return hi;
+  const int stride = (L + 63) >> 6;
+  int idx = lo + (lane + 1) * stride - 1;
+  idx = idx < hi - 1 ? idx : hi - 1;
+  const unsigned long long b1 = __ballot(a[idx] >= thr);
+  if (b1 == 0) return hi;
+  const int f = __ffsll(static_cast<long long>(b1)) - 1;
+  const int base = lo + f * stride;
+  const int j = base + lane;
+  const bool ok = j < hi && j < base + stride && a[j] >= thr;
+  const unsigned long long b2 = __ballot(ok);
+  return base + __ffsll(static_cast<long long>(b2)) - 1;
 }
 
 // ---- select: one block ------------------------------------------------------
 // Candidates are taken from the top of the stack in windows of kSelWin.  The
 // inclusive prefix sums P of the reservations 2(n - depth) (0 for parents
-// whose bound is not below the incumbent) locate every sub-batch boundary by
-// binary search: a sub-batch that opened at candidate b (with carried
-// reservation c) closes in front of the first candidate i with
-// c + P[i] - P[b-1] >= pool_size - 1, which is the parent push_back1 turns
-// away and the next sub-batch starts with.
+// whose bound is not below the incumbent) locate every sub-batch boundary: a
+// sub-batch that opened at candidate b (with carried reservation c) closes in
+// front of the first candidate i with c + P[i] - P[b-1] >= pool_size - 1,
+// which is the parent push_back1 turns away and the next sub-batch starts
+// with.  Wave 0 walks the boundaries with wave-wide searches; the taken
+// parents and their first-child offsets are then written by all threads.
 __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   __shared__ int s_p[kSelWin];  // prefix sums of reservations
   __shared__ int s_c[kSelWin];  // children generated per candidate
   __shared__ int s_w16[40];
-  __shared__ int s_state[8];
-  __shared__ int s_tmp[1024];
+  __shared__ int s_state[4];
   Ctrl* C = S.ctrl;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   constexpr int kPer = kSelWin / 1024;
   if (C->status != kRunning) {
     if (tid == 0) {
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   }
   const int inc = C->inc;
   // block-uniform carried state: reservation of the open sub-batch, closed
-  // sub-batches, parents taken, pruned candidates, candidates consumed
-  int carry = 0, nb = 0, np = 0, pruned = 0, consumed = 0;
+  // sub-batches, parents taken, children, pruned candidates, consumed
+  int carry = 0, nb = 0, np = 0, nc = 0, pruned = 0, consumed = 0;
   for (;;) {
     const int win = min(kSelWin, top - consumed);
     if (win <= 0) break;
@@ -226,72 +226,72 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     const int off = block_exscan1024(tsum, s_w16, &total);
     for (int k = 0; k < kPer; ++k) s_p[kPer * tid + k] = off + loc[k];
     __syncthreads();
-    if (tid == 0) {
-      // first candidate whose prefix reaches `thr`, searching [lo, win)
-      auto first_at_least = [&](int lo, int thr) {
-        int hi = win;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_p[mid] >= thr) hi = mid;
-          else lo = mid + 1;
-        }
-        return lo;
-      };
-      int end = win, stop = 0, base = 0, cur = carry;  // P[b-1] of the open sub-batch
-      int i = first_at_least(0, S.pool_m1 - cur);
+    if (tid < 64) {
+      int end = win, stop = 0, base = 0, cur = carry, nbw = nb;
+      int i = wave_first_at_least(s_p, 0, win, S.pool_m1 - cur, lane);
       while (i < win) {
-        ++nb;
-        if (nb == S.max_batches) {
+        ++nbw;
+        if (nbw == S.max_batches) {
           stop = 1;
           end = i;
           break;
         }
         base = i > 0 ? s_p[i - 1] : 0;
         cur = 0;
-        i = first_at_least(i + 1, base + S.pool_m1);
+        i = wave_first_at_least(s_p, i + 1, win, base + S.pool_m1, lane);
       }
-      const int lastp = win > 0 ? s_p[win - 1] : 0;
-      s_state[0] = end;
-      s_state[1] = stop;
-      s_state[2] = cur + (stop ? 0 : lastp - base);  // reservation carried on
+      if (lane == 0) {
+        const int lastp = s_p[win - 1];
+        s_state[0] = end;
+        s_state[1] = stop;
+        s_state[2] = cur + (stop ? 0 : lastp - base);  // reservation carried on
+        s_state[3] = nbw;
+      }
     }
     __syncthreads();
     const int end = s_state[0], stop = s_state[1];
     carry = s_state[2];
+    nb = s_state[3];
     // parents taken in [0, end): candidates whose prefix steps up
-    int mine = 0;
+    int mine = 0, mc = 0;
     for (int k = 0; k < kPer; ++k) {
       const int i = kPer * tid + k;
       const int prev = i > 0 ? s_p[i - 1] : 0;
-      mine += (i < end && s_p[i] > prev);
+      const bool t = i < end && s_p[i] > prev;
+      mine += t;
+      mc += t ? s_c[i] : 0;
     }
-    int ntaken;
+    int ntaken, nchild;
     int slot = np + block_exscan1024(mine, s_w16, &ntaken);
+    int coff = nc + block_exscan1024(mc, s_w16, &nchild);
     for (int k = 0; k < kPer; ++k) {
       const int i = kPer * tid + k;
       const int prev = i > 0 ? s_p[i - 1] : 0;
       if (i < end && s_p[i] > prev) {
         if (slot < S.PCAP) {
           S.p_sel[slot] = top - 1 - (consumed + i);
-          S.p_cnt[slot] = s_c[i];
+          S.p_off[slot] = coff;
         }
         ++slot;
+        coff += s_c[i];
       }
     }
     np += ntaken;
+    nc += nchild;
     pruned += end - ntaken;
     consumed += end;
     __syncthreads();
     if (stop) break;
   }
-  if (np > S.PCAP) {
-    if (tid == 0) C->status = kOverflow;
-    return;
-  }
-  const int nc = block_scan_excl(S.p_cnt, S.p_off, np, 0, OpAdd(), s_tmp);
   if (tid == 0) {
-    S.p_off[np] = nc;
     C->stats[kStRounds] += 1;
+    if (np > S.PCAP || nc > S.CCAP) {
+      C->status = kOverflow;
+      C->n_parents = 0;
+      C->n_children = 0;
+      return;
+    }
+    S.p_off[np] = nc;
     C->stats[kStPrunedPop] += pruned;
     C->stack_size = top - consumed;
     if (S.budget > 0 && C->stats[kStDecomposed] + np > S.budget) {
@@ -307,45 +307,73 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
       C->n_children = nc;
       C->stats[kStDecomposed] += np;
       C->stats[kStBounded] += nc;
+      if (S.count_pairs) C->stats[kStPairs] += static_cast<long long>(nc) * S.P;
     }
-    C->best_child = -1;
   }
 }
 
 // ---- expand: parents out of the stack, child descriptors ------------------
+// A thread per parent copies its permutation column (consecutive parents sit
+// at consecutive stack slots, so every position row is read coalesced); the
+// wave then writes the child descriptors of its 64 parents one parent at a
+// time, lanes over consecutive children.
 __global__ __launch_bounds__(256) void k_expand(SArgs S) {
   const Ctrl* C = S.ctrl;
   const int np = C->n_parents;
   const int n = S.n, m = S.m, P = S.P;
+  const int lane = threadIdx.x & 63;
   long long work = 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-    const int si = S.p_sel[p];
-    for (int i = 0; i < n; ++i)
-      S.p_perm[static_cast<size_t>(i) * S.PCAP + p] = S.s_perm[static_cast<size_t>(i) * S.CAP + si];
-    const int l1 = S.s_l1[si], l2 = S.s_l2[si];
-    S.p_l1[p] = l1;
-    S.p_l2[p] = l2;
-    S.p_cost[p] = S.s_cost[si];
-    int off = S.p_off[p];
-    const int depth = (l1 + 1) + (n - l2);
-    const int tag = p << 9;
-    // algorithmic bound work of the children: per child with prefix d1,
-    // suffix d2 and u free jobs, P(n + 4u + 4) + 2m d1 + m(m+1) d2 int ops
-    const long long d1 = l1 + 1, d2 = n - l2, u = n - depth;
-    const long long mm1 = static_cast<long long>(m) * (m + 1);
-    if (depth == n - 2) {  // simple parent: two leaves, BEGIN side
-      S.c_desc[off] = tag | (l1 + 1);
-      S.c_desc[off + 1] = tag | (l1 + 2);
-      work += 2 * (P * (n + 4LL) + 2LL * m * (d1 + 2) + mm1 * d2);
-    } else {
-      for (int side = 0; side < 2; ++side)
-        for (int j = l1 + 1; j < l2; ++j) S.c_desc[off++] = tag | (side << 8) | j;
-      const long long base = P * (n + 4 * (u - 1) + 4);
-      work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
+  for (int pb = blockIdx.x * blockDim.x; pb < np; pb += gridDim.x * blockDim.x) {
+    const int p = pb + threadIdx.x;
+    int l1 = 0, l2 = 0, off = 0;
+    if (p < np) {
+      const int si = S.p_sel[p];
+      // loads of a chunk all issue before its stores (the arrays may alias
+      // as far as the compiler knows)
+      for (int i0 = 0; i0 < n; i0 += 16) {
+        uint8_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) v[u] = S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + si];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
+      }
+      l1 = S.s_l1[si];
+      l2 = S.s_l2[si];
+      S.p_l1[p] = l1;
+      S.p_l2[p] = l2;
+      S.p_cost[p] = S.s_cost[si];
+      off = S.p_off[p];
+      // algorithmic bound work of the children: per child with prefix d1,
+      // suffix d2 and u free jobs, P(n + 4u + 4) + 2m d1 + m(m+1) d2 int ops
+      const long long d1 = l1 + 1, d2 = n - l2, u = l2 - l1 - 1;
+      const long long mm1 = static_cast<long long>(m) * (m + 1);
+      if (d1 + d2 == n - 2) {
+        work += 2 * (P * (n + 4LL) + 2LL * m * (d1 + 2) + mm1 * d2);
+      } else {
+        const long long base = P * (n + 4 * (u - 1) + 4);
+        work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
+      }
+    }
+    const int wbase = pb + (threadIdx.x & ~63);
+    const int cntw = min(64, np - wbase);  // wave-uniform
+    for (int q = 0; q < cntw; ++q) {
+      const int ql1 = __shfl(l1, q, 64), ql2 = __shfl(l2, q, 64), qoff = __shfl(off, q, 64);
+      const int tag = (wbase + q) << 9;
+      const int u = ql2 - ql1 - 1;
+      if ((ql1 + 1) + (n - ql2) == n - 2) {  // simple parent: two leaves, BEGIN side
+        if (lane < 2) S.c_desc[qoff + lane] = tag | (ql1 + 1 + lane);
+      } else {                               // BEGIN children then END children
+        for (int c = lane; c < 2 * u; c += 64) {
+          const int side = c >= u;
+          S.c_desc[qoff + c] = tag | (side << 8) | (ql1 + 1 + c - side * u);
+        }
+      }
     }
   }
   for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
-  if ((threadIdx.x & 63) == 0 && work)
+  if (lane == 0 && work)
     atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStWork]),
               static_cast<unsigned long long>(work));
 }
@@ -358,6 +386,7 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
   const int np = C->n_parents;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = S.n;
+  const int inc0 = C->inc;
   long long leaves = 0;
   for (int pb = blockIdx.x * 4; pb < np; pb += gridDim.x * 4) {
     const int p = pb + wave;
@@ -401,7 +430,7 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
     __syncthreads();
     if (live) {
       const int first = simple ? 0 : side * u;
-      int lmin = INT_MAX, lwin = -1, lwjob = -1, lmax = INT_MIN;
+      int lmin = INT_MAX, lwin = -1, lwjob = -1, lmax = INT_MIN, below = 0;
       for (int c = lane; c < cnt; c += 64) {
         const int i = c - first;
         if (i < 0 || i >= nch) {
@@ -416,6 +445,7 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
         }
         S.c_rank[off + c] = rk;
         S.c_scost[off + rk] = cc;
+        below += cc < inc0;
         if (simple) {
           // leaves are inserted worst first (cost desc, job desc) and only a
           // strictly better one replaces the incumbent: among equal minima
@@ -440,7 +470,9 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
           lmax = max(lmax, __shfl_xor(lmax, d, 64));
         }
       }
+      for (int d = 32; d >= 1; d >>= 1) below += __shfl_xor(below, d, 64);
       if (lane == 0) {
+        S.p_kept0[p] = below;
         S.p_side[p] = side;
         S.p_nch[p] = nch;
         S.p_leafmin[p] = simple ? lmin : INT_MAX;
@@ -455,95 +487,123 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
                                      static_cast<unsigned long long>(leaves));
 }
 
-// ---- incumbent seen by each parent (running min of earlier leaves) --------
-__global__ __launch_bounds__(1024) void k_incumbent(SArgs S) {
-  __shared__ int s_tmp[1024];
+// ---- settle: incumbent per parent, kept children, survivor offsets --------
+// One block, parents in chunks of 4096 (thread t owns four consecutive
+// parents).  The incumbent a parent's children are pruned against is the
+// running minimum of the leaves of the parents popped before it
+// (solutions::insert strictly-better rule, solutions.cpp:45-65; prune loop
+// gpu.cu:317-394).  Its kept children are the ranks whose bound is below
+// that incumbent: counted by the rank kernel against the round's starting
+// incumbent, searched again only for parents after an improvement.  Prefix
+// sums of the kept counts place the survivors on the stack.
+__global__ __launch_bounds__(1024) void k_settle(SArgs S) {
+  constexpr int K = 4;
+  __shared__ int s_w16[40];
   __shared__ int s_last;
   Ctrl* C = S.ctrl;
   const int np = C->n_parents;
   if (np == 0) return;
+  const int n = S.n, tid = threadIdx.x;
   const int inc0 = C->inc;
-  block_scan_excl(S.p_leafmin, S.p_incb, np, INT_MAX, OpMin(), s_tmp);
-  if (threadIdx.x == 0) s_last = -1;
-  __syncthreads();
+  if (tid == 0) s_last = -1;
+  int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
   int improved = 0, last = -1;
-  for (int p = threadIdx.x; p < np; p += blockDim.x) {
-    const int ib = min(inc0, S.p_incb[p]);
-    S.p_incb[p] = ib;
-    const int lmin = S.p_leafmin[p];
-    if (lmin < ib) {
-      // two leaves, inserted worst first: the first may improve, then the
-      // second only if strictly better still
-      const int lmax = S.p_leafmax[p];
-      improved += (lmax < ib) + (lmin < min(ib, lmax));
-      last = max(last, p);
-    }
-  }
-  atomicMax(&s_last, last);
-  if (improved) atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStImproved]),
-                          static_cast<unsigned long long>(improved));
-  __syncthreads();
-  if (threadIdx.x == 0 && s_last >= 0) {
-    const int p = s_last;
-    C->inc = S.p_leafmin[p];
-    C->best_child = S.p_leafwin[p];
-  }
-}
-
-// ---- kept children per parent ---------------------------------------------
-__global__ __launch_bounds__(256) void k_keep(SArgs S) {
-  const Ctrl* C = S.ctrl;
-  const int np = C->n_parents;
-  const int n = S.n;
   long long pruned = 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-    const int l1 = S.p_l1[p], l2 = S.p_l2[p];
-    const bool simple = ((l1 + 1) + (n - l2)) == n - 2;
-    int kept = 0;
-    if (!simple) {
-      const int off = S.p_off[p], nch = S.p_nch[p], lim = S.p_incb[p];
-      int lo = 0, hi = nch;  // first rank whose bound is >= lim
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (S.c_scost[off + mid] < lim) lo = mid + 1;
-        else hi = mid;
-      }
-      kept = lo;
-      pruned += nch - kept;
+  for (int c0 = 0; c0 < np; c0 += 1024 * K) {
+    int lmin[K], kept[K];
+    int tmin = INT_MAX;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int p = c0 + K * tid + k;
+      lmin[k] = p < np ? S.p_leafmin[p] : INT_MAX;
+      tmin = min(tmin, lmin[k]);
     }
-    S.p_kept[p] = kept;
+    int cmin;
+    int ib = min(run_min, block_exscan1024(tmin, INT_MAX, OpMin(), s_w16, &cmin));
+    int tkept = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int p = c0 + K * tid + k;
+      kept[k] = 0;
+      if (p < np) {
+        if (lmin[k] < ib) {
+          // two leaves, inserted worst first: the first may improve, then
+          // the second only if strictly better still
+          const int lmax = S.p_leafmax[p];
+          improved += (lmax < ib) + (lmin[k] < min(ib, lmax));
+          last = p;
+        }
+        const int l1 = S.p_l1[p], l2 = S.p_l2[p];
+        if (((l1 + 1) + (n - l2)) != n - 2) {
+          const int nch = S.p_nch[p];
+          int kp;
+          if (ib == inc0) {
+            kp = S.p_kept0[p];
+          } else {
+            const int off = S.p_off[p];
+            int lo = 0, hi = nch;  // first rank whose bound is >= ib
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (S.c_scost[off + mid] < ib) lo = mid + 1;
+              else hi = mid;
+            }
+            kp = lo;
+          }
+          kept[k] = kp;
+          pruned += nch - kp;
+        }
+        tkept += kept[k];
+      }
+      ib = min(ib, lmin[k]);
+    }
+    int csum;
+    int koff = run_kept + block_exscan1024(tkept, s_w16, &csum);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int p = c0 + K * tid + k;
+      if (p < np) {
+        S.p_kept[p] = kept[k];
+        S.p_koff[p] = koff;
+        koff += kept[k];
+      }
+    }
+    run_min = min(run_min, cmin);
+    run_kept += csum;
   }
-  for (int d = 32; d >= 1; d >>= 1) pruned += __shfl_xor(pruned, d, 64);
-  if ((threadIdx.x & 63) == 0 && pruned)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStPrunedIns]),
-              static_cast<unsigned long long>(pruned));
-}
-
-// ---- survivor offsets, capacity check, best permutation -------------------
-__global__ __launch_bounds__(1024) void k_offsets(SArgs S) {
-  __shared__ int s_tmp[1024];
-  Ctrl* C = S.ctrl;
-  const int np = C->n_parents;
-  if (np == 0) return;
-  const int total = block_scan_excl(S.p_kept, S.p_koff, np, 0, OpAdd(), s_tmp);
-  if (threadIdx.x == 0) {
+  if (last >= 0) atomicMax(&s_last, last);
+  for (int d = 32; d >= 1; d >>= 1) {
+    improved += __shfl_xor(improved, d, 64);
+    pruned += __shfl_xor(pruned, d, 64);
+  }
+  if ((tid & 63) == 0) {
+    if (improved)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStImproved]),
+                static_cast<unsigned long long>(improved));
+    if (pruned)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStPrunedIns]),
+                static_cast<unsigned long long>(pruned));
+  }
+  __syncthreads();
+  const int bl = s_last;
+  if (tid == 0) {
+    if (bl >= 0) C->inc = run_min;
     const int base = C->stack_size;
     C->push_base = base;
-    C->push_total = total;
-    if (static_cast<long long>(base) + total > S.CAP) {
+    C->push_total = run_kept;
+    if (static_cast<long long>(base) + run_kept > S.CAP) {
       C->status = kOverflow;
     } else {
-      C->stack_size = base + total;
-      if (base + total > C->stats[kStMaxPool]) C->stats[kStMaxPool] = base + total;
+      C->stack_size = base + run_kept;
+      if (base + run_kept > C->stats[kStMaxPool]) C->stats[kStMaxPool] = base + run_kept;
     }
   }
   // the leaf that set the incumbent: parent permutation with its swap
-  const int bc = C->best_child;
-  if (bc >= 0 && threadIdx.x < S.n) {
+  if (bl >= 0 && tid < n) {
+    const int bc = S.p_leafwin[bl];
     const int d = S.c_desc[bc];
     const int p = d >> 9, side = (d >> 8) & 1, j = d & 255;
     const int sw0 = side == 0 ? S.p_l1[p] + 1 : S.p_l2[p] - 1;
-    const int i = threadIdx.x;
+    const int i = tid;
     const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
     C->best_perm[i] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
     if (i == 0) C->have_perm = 1;
@@ -569,9 +629,17 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
     int l1 = pl1 + (side == 0), l2 = pl2 - (side == 1);
     if (l2 - l1 == 2) l1 += 1;
     const int sw0 = side == 0 ? pl1 + 1 : pl2 - 1;
-    for (int i = 0; i < n; ++i) {
-      const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
-      S.s_perm[static_cast<size_t>(i) * S.CAP + slot] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
+    for (int i0 = 0; i0 < n; i0 += 16) {
+      uint8_t v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int i = i0 + u;
+        const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
+        if (i < n) v[u] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (i0 + u < n) S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + slot] = v[u];
     }
     S.s_l1[slot] = l1;
     S.s_l2[slot] = l2;
@@ -594,7 +662,6 @@ __global__ void k_init(SArgs S, int ub0) {
     C->stack_size = 1;
     C->inc = ub0;
     C->n_parents = C->n_children = 0;
-    C->best_child = -1;
     C->have_perm = 0;
   }
 }
@@ -632,9 +699,6 @@ __global__ void k_log_advance(LogArgs L) {
   *L.log_count = min(L.cap, *L.log_count + nrec);
 }
 
-__global__ void k_count_pairs(SArgs S) {
-  S.ctrl->stats[kStPairs] += static_cast<long long>(S.ctrl->n_children) * S.P;
-}
 
 }  // namespace
 
@@ -712,8 +776,8 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_sl2 = take(4 * size_t(cap_nodes));
   const size_t o_scost = take(4 * size_t(cap_nodes));
   const size_t o_pperm = take(size_t(n) * PCAP);
-  size_t o_pint[14];
-  for (int i = 0; i < 14; ++i) o_pint[i] = take(4 * size_t(PCAP + 1));
+  size_t o_pint[13];
+  for (int i = 0; i < 13; ++i) o_pint[i] = take(4 * size_t(PCAP + 1));
   size_t o_cint[4];
   for (int i = 0; i < 4; ++i) o_cint[i] = take(4 * size_t(CCAP));
   hipError_t e = S->mem.reserve(off);
@@ -740,10 +804,10 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.s_l2 = S->mem.as<int>(o_sl2);
   A.s_cost = S->mem.as<int>(o_scost);
   A.p_perm = S->mem.as<uint8_t>(o_pperm);
-  int** pints[14] = {&A.p_l1,      &A.p_l2,      &A.p_cost,    &A.p_sel,  &A.p_cnt,
+  int** pints[13] = {&A.p_l1,      &A.p_l2,      &A.p_cost,    &A.p_sel,
                      &A.p_off,     &A.p_side,    &A.p_nch,     &A.p_leafmin, &A.p_leafmax,
-                     &A.p_leafwin, &A.p_incb,    &A.p_kept,    &A.p_koff};
-  for (int i = 0; i < 14; ++i) *pints[i] = S->mem.as<int>(o_pint[i]);
+                     &A.p_leafwin, &A.p_kept0,   &A.p_kept,    &A.p_koff};
+  for (int i = 0; i < 13; ++i) *pints[i] = S->mem.as<int>(o_pint[i]);
   int** cints[4] = {&A.c_desc, &A.c_bound, &A.c_rank, &A.c_scost};
   for (int i = 0; i < 4; ++i) *cints[i] = S->mem.as<int>(o_cint[i]);
   return WSBB_OK;
@@ -752,11 +816,11 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
 static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent_t ev0,
                                hipEvent_t ev1) {
   hipStream_t st = ctx->stream;
-  const int pblocks = (A.PCAP + 255) / 256;
+  const int eblocks = (A.PCAP + 255) / 256;
   const int cblocks = (A.CCAP + 255) / 256;
   const int rblocks = (A.PCAP + 3) / 4;
   hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
-  hipLaunchKernelGGL(k_expand, dim3(pblocks), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_expand, dim3(eblocks), dim3(256), 0, st, A);
   ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
   SearchState* SS = ctx->search;
   const int log_stride = selector_log_stride(ctx);
@@ -765,7 +829,16 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   B.Ndev = &A.ctrl->n_children;
   B.kind = kind == WSBB_LB1 ? 1 : 0;
   B.best = -1;
-  B.groups = ctx->groups_for(A.CCAP);
+  // a round's live child count is often well below CCAP: split pairs over
+  // more waves than a full batch of CCAP would need (measured on 20x10:
+  // four waves per tile beat one, two, eight and sixteen)
+  {
+    const int tiles = (A.CCAP + 63) / 64, simds = 4 * ctx->num_cus;
+    B.groups = ctx->launch.groups > 1 ? ctx->launch.groups
+               : tiles < 2 * simds    ? 4
+               : tiles < 4 * simds    ? 2
+                                      : 1;
+  }
   B.out = A.c_bound;
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
   hipError_t e;
@@ -797,7 +870,6 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
   if (e != hipSuccess) return e;
   if (ev1) (void)hipEventRecord(ev1, st);
-  if (kind != WSBB_LEARNED) hipLaunchKernelGGL(k_count_pairs, dim3(1), dim3(1), 0, st, A);
   if (kind != WSBB_LEARNED && log_stride > 0) {
     LogArgs L{};
     L.ctrl = A.ctrl;
@@ -813,9 +885,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     }
   }
   hipLaunchKernelGGL(k_rank, dim3(rblocks < 4096 ? rblocks : 4096), dim3(256), 0, st, A);
-  hipLaunchKernelGGL(k_incumbent, dim3(1), dim3(1024), 0, st, A);
-  hipLaunchKernelGGL(k_keep, dim3(pblocks), dim3(256), 0, st, A);
-  hipLaunchKernelGGL(k_offsets, dim3(1), dim3(1024), 0, st, A);
+  hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
   hipLaunchKernelGGL(k_push, dim3(cblocks), dim3(256), 0, st, A);
   return hipGetLastError();
 }
@@ -850,6 +920,7 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   A.pool_m1 = pool_size - 1;
   A.max_batches = max_batches;
   A.budget = budget;
+  A.count_pairs = kind == WSBB_LB2;
   hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");
   const bool timing = ctx->profile_bound;
