@@ -170,6 +170,36 @@ int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
                         int64_t *bound_launches);
 /* Device node-pool capacity in nodes (default 2^24) for later solves. */
 int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
+/* Launch options of later solves: use_graph = 1 captures the kernels of
+ * rounds_per_sync rounds in a hipGraph once per solve and replays it, 0
+ * launches them directly (-1 keeps the setting; default 1); rounds_per_sync
+ * rounds run between host checks of the pool state (1..64, -1 keeps it;
+ * default 8).  Bound-kernel timing (wsbb_search_profile) launches directly.
+ * No counterpart in the reference (its host loop synchronises every pool,
+ * explorer.cu:216-250). */
+int wsbb_search_options(wsbb_ctx *ctx, int use_graph, int rounds_per_sync);
+/* Checkpoint of the last solve's device state: incumbent, counters, the
+ * incumbent's permutation (-1s if none) and the node pool left when it
+ * stopped (budget, overflow) -- the count of pool nodes in *count, and with
+ * buffers of cap >= count nodes: permT [n][count] (position-major), l1, l2,
+ * cost [count].  Counterpart of solutions::save (src/solutions.cpp:97-108),
+ * extended to the pool. */
+int wsbb_search_state(wsbb_ctx *ctx, int32_t *inc, int64_t *stats,
+                      int32_t *best_perm, int *count, uint8_t *permT,
+                      int32_t *l1, int32_t *l2, int32_t *cost, int cap);
+/* Continues a search from a saved pool (same layout as wsbb_search_state),
+ * incumbent inc, counters stats_in (may be NULL) and incumbent permutation
+ * best_perm_in (may be NULL); other arguments and outputs as wsbb_solve.  A
+ * search stopped by its budget and resumed here visits exactly the nodes an
+ * uninterrupted search visits.  Counterpart of solutions::init /
+ * readOptimalSolution (src/solutions.cpp:24-33, 69-86), extended to the
+ * pool. */
+int wsbb_solve_resume(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
+                      int64_t budget, const uint8_t *permT, const int32_t *l1,
+                      const int32_t *l2, const int32_t *cost, int count,
+                      int32_t inc, const int64_t *stats_in,
+                      const int32_t *best_perm_in, int32_t *best,
+                      int32_t *best_perm, int64_t *stats);
 
 /* ---- learned machine-pair selector -------------------------------------------
  * No counterpart in the reference (its pair order ordoSomme is fixed,

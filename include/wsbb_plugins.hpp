@@ -10,12 +10,18 @@
 //                               src/decompose_beginend.cpp:26-107)
 // and adds bound_hip, the GPU bound behind bound_abstract.  A per-node
 // bornes_calculer is a batch of one; bornes_calculer_batch bounds a whole
-// vector in one launch, which is how decompose_beginend uses it.
+// vector in one launch, which is how decompose_beginend uses it.  It also
+// reads and writes the reference's text formats (node records, bab.solutions
+// blocks, the formater CSV row) and a pool checkpoint file of the device
+// search built from those records.
 #ifndef WSBB_PLUGINS_HPP
 #define WSBB_PLUGINS_HPP
 
 #include <algorithm>
 #include <cstdint>
+#include <fstream>
+#include <istream>
+#include <ostream>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -234,6 +240,151 @@ class decompose_beginend {
   int best_ = 99999;
   problems ns_;
 };
+
+// ---- text formats ------------------------------------------------------------
+
+// A node as text, in the field order and spacing of raw_bb_problem's
+// operator<< / operator>> (bb_problem_calculator.cpp:101-144): couts[0]
+// couts[1] couts_somme, `njobs` permutation entries (the reference writes
+// MAX_NBJOBS = 20), limite1 limite2 depth job begin_end, each followed by a
+// space, then a newline.
+inline std::ostream& write_record(std::ostream& os, const raw_bb_problem& p, int njobs = 20) {
+  os << p.couts[0] << ' ' << p.couts[1] << ' ' << p.couts_somme << ' ';
+  for (int j = 0; j < njobs; ++j) os << p.permutation[j] << ' ';
+  os << p.limite1 << ' ' << p.limite2 << ' ' << p.depth << ' ' << p.job << ' ' << p.begin_end
+     << ' ' << '\n';
+  return os;
+}
+inline bool read_record(std::istream& is, raw_bb_problem& p, int njobs = 20) {
+  if (njobs < 0 || njobs > MAX_NBJOBS) return false;
+  is >> p.couts[0] >> p.couts[1] >> p.couts_somme;
+  for (int j = 0; j < njobs; ++j) is >> p.permutation[j];
+  is >> p.limite1 >> p.limite2 >> p.depth >> p.job >> p.begin_end;
+  return static_cast<bool>(is);
+}
+
+// One bab.solutions block (operator<< for solutions, solutions.cpp:88-95):
+// the count, a space and a newline, then the records.  solutions::save
+// (97-108) appends a block at every improvement.
+inline void write_solutions(std::ostream& os, const std::vector<raw_bb_problem>& s,
+                            int njobs = 20) {
+  os << s.size() << ' ' << '\n';
+  for (const auto& r : s) write_record(os, r, njobs);
+}
+inline void save_solutions(const std::string& path, const std::vector<raw_bb_problem>& s,
+                           int njobs = 20) {
+  std::ofstream f(path, std::ofstream::app);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  write_solutions(f, s, njobs);
+}
+// the first block of a bab.solutions stream (readOptimalSolution 69-86)
+inline std::vector<raw_bb_problem> read_solutions(std::istream& is, int njobs = 20) {
+  int count = 0;
+  std::vector<raw_bb_problem> out;
+  if (!(is >> count) || count < 0) return out;
+  for (int i = 0; i < count; ++i) {
+    raw_bb_problem r{};
+    if (!read_record(is, r, njobs)) throw std::runtime_error("truncated solutions block");
+    out.push_back(r);
+  }
+  return out;
+}
+
+// One row of the formater's CSV (formater.cpp:94-116):
+// "instance;makespan;nodes;time;"
+inline std::string formater_line(int instance, int makespan, long long nodes, double seconds) {
+  std::ostringstream os;
+  os << instance << ';' << makespan << ';' << nodes << ';' << static_cast<float>(seconds) << ';';
+  return os.str();
+}
+
+// ---- device pool checkpoint -------------------------------------------------
+// A text file: "wsbb-pool <n> <incumbent> <count>", the counters, the
+// incumbent's permutation (-1s if none), then one record per pool node
+// (njobs = n; couts[0] = couts_somme = the node's bound, depth from its
+// limits, job and begin_end -1), deepest-first as the stack holds them.
+struct pool_snapshot {
+  int n = 0;
+  int32_t inc = 0;
+  std::vector<int64_t> stats;
+  std::vector<int32_t> best_perm;
+  std::vector<uint8_t> permT;  // [n][count]
+  std::vector<int32_t> l1, l2, cost;
+  int count() const { return static_cast<int>(l1.size()); }
+};
+
+inline void save_pool(std::ostream& os, const pool_snapshot& s) {
+  os << "wsbb-pool " << s.n << ' ' << s.inc << ' ' << s.count() << '\n';
+  for (int64_t v : s.stats) os << v << ' ';
+  os << '\n';
+  for (int32_t v : s.best_perm) os << v << ' ';
+  os << '\n';
+  const int c = s.count();
+  for (int k = 0; k < c; ++k) {
+    raw_bb_problem r{};
+    r.couts[0] = r.couts_somme = s.cost[k];
+    for (int i = 0; i < s.n; ++i) r.permutation[i] = s.permT[static_cast<size_t>(i) * c + k];
+    r.limite1 = s.l1[k];
+    r.limite2 = s.l2[k];
+    r.depth = (s.l1[k] + 1) + (s.n - s.l2[k]);
+    r.job = r.begin_end = -1;
+    write_record(os, r, s.n);
+  }
+}
+inline pool_snapshot load_pool(std::istream& is) {
+  pool_snapshot s;
+  std::string tag;
+  int count = 0;
+  is >> tag >> s.n >> s.inc >> count;
+  if (!is || tag != "wsbb-pool" || s.n < 2 || s.n > MAX_NBJOBS || count < 0)
+    throw std::runtime_error("not a wsbb-pool file");
+  s.stats.resize(WSBB_NSTATS);
+  for (auto& v : s.stats) is >> v;
+  s.best_perm.resize(s.n);
+  for (auto& v : s.best_perm) is >> v;
+  s.permT.resize(static_cast<size_t>(s.n) * count);
+  s.l1.resize(count);
+  s.l2.resize(count);
+  s.cost.resize(count);
+  for (int k = 0; k < count; ++k) {
+    raw_bb_problem r{};
+    if (!read_record(is, r, s.n)) throw std::runtime_error("truncated wsbb-pool file");
+    for (int i = 0; i < s.n; ++i) s.permT[static_cast<size_t>(i) * count + k] = static_cast<uint8_t>(r.permutation[i]);
+    s.l1[k] = r.limite1;
+    s.l2[k] = r.limite2;
+    s.cost[k] = r.couts[0];
+  }
+  return s;
+}
+
+// the last solve's state of a context, as a snapshot
+inline pool_snapshot snapshot(wsbb_ctx* ctx) {
+  pool_snapshot s;
+  int n = 0, m = 0, P = 0, cus = 0, count = 0;
+  if (wsbb_info(ctx, &n, &m, &P, &cus) != WSBB_OK) throw std::runtime_error(wsbb_last_error());
+  s.n = n;
+  s.stats.resize(WSBB_NSTATS);
+  s.best_perm.resize(n);
+  if (wsbb_search_state(ctx, &s.inc, s.stats.data(), s.best_perm.data(), &count, nullptr,
+                        nullptr, nullptr, nullptr, 0) != WSBB_OK)
+    throw std::runtime_error(wsbb_last_error());
+  s.permT.resize(static_cast<size_t>(n) * count);
+  s.l1.resize(count);
+  s.l2.resize(count);
+  s.cost.resize(count);
+  if (wsbb_search_state(ctx, &s.inc, s.stats.data(), s.best_perm.data(), &count, s.permT.data(),
+                        s.l1.data(), s.l2.data(), s.cost.data(), count) != WSBB_OK)
+    throw std::runtime_error(wsbb_last_error());
+  return s;
+}
+
+// continue a search from a snapshot; returns wsbb_solve_resume's code
+inline int resume(wsbb_ctx* ctx, const pool_snapshot& s, int kind, int pool_size, int max_batches,
+                  int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats) {
+  return wsbb_solve_resume(ctx, kind, pool_size, max_batches, budget, s.permT.data(), s.l1.data(),
+                           s.l2.data(), s.cost.data(), s.count(), s.inc, s.stats.data(),
+                           s.best_perm.data(), best, best_perm, stats);
+}
 
 // the root of the reference's tree (create_raw_problem 9-22, add_root)
 inline raw_bb_problem* make_root(int size) {

@@ -25,3 +25,22 @@ def test_plugin_explorer_matches_oracle_dfs(tid):
     assert best == ref
     assert (decomposed, bounded, leaves, pruned) == (
         st["decomposed"], st["bounded"], st["leaves"], st["pruned_pop"])
+
+
+def test_pool_checkpoint_file_resume(tmp_path):
+    """A device search saved to a pool checkpoint file mid-way and resumed
+    from the file in a fresh context ends with the uninterrupted search's
+    optimum and counters."""
+    import wsbb_amd
+    assert os.path.exists(BIN), "build the library first (wsbb_amd/build.py)"
+    path = str(tmp_path / "ta011.pool")
+    out = subprocess.run([BIN, "--resume", "11", "100000", path], capture_output=True, text=True,
+                         timeout=240, check=True).stdout.split()
+    best, proven, st = int(out[0]), int(out[1]), list(map(int, out[2:]))
+    with wsbb_amd.Engine(oracle.taillard(11)[3]) as eng:
+        full = eng.solve(pool_size=4096, max_batches=10)
+    assert proven == 1 and best == full["best"] == 1582
+    for i, k in enumerate(wsbb_amd.STAT_NAMES[:8]):
+        assert st[i] == full["stats"][k], k
+    with open(path) as f:
+        assert f.readline().startswith("wsbb-pool 20 ")

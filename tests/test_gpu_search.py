@@ -82,3 +82,48 @@ def test_budget_stops_without_proof():
     assert res["stats"]["decomposed"] <= 500
     if res["perm"][0] >= 0:
         assert cmax_np(p, res["perm"]) == res["best"]
+
+
+@pytest.mark.parametrize("rounds", [1, 3, 16])
+def test_graph_replay_matches_direct_launches(rounds):
+    """Rounds replayed from a captured hipGraph (many replays per solve, host
+    checks between them) give the same search as direct launches, which the
+    tests above compare with the oracle."""
+    for tid, pool, nb in ((1, 4096, 1), (11, 4096, 10), (3, 300, 3)):
+        p = oracle.taillard(tid)[3]
+        with wsbb_amd.Engine(p) as eng:
+            eng.search_options(use_graph=0, rounds_per_sync=rounds)
+            direct = eng.solve(pool_size=pool, max_batches=nb)
+            eng.search_options(use_graph=1)
+            graph = eng.solve(pool_size=pool, max_batches=nb)
+            again = eng.solve(pool_size=pool, max_batches=nb)
+        for res in (graph, again):
+            assert res["best"] == direct["best"] and res["proven"]
+            for k in STATS:
+                assert res["stats"][k] == direct["stats"][k], (tid, k)
+
+
+def test_checkpoint_resume_continues_exactly():
+    """A search stopped by its node budget, saved (pool, incumbent, counters)
+    and resumed in a fresh context, in segments, visits exactly the nodes of
+    the uninterrupted search."""
+    p = oracle.taillard(11)[3]
+    with wsbb_amd.Engine(p) as eng:
+        full = eng.solve(pool_size=4096, max_batches=10)
+    ck = None
+    res = None
+    for seg_budget in (60_000, 150_000, 0):
+        with wsbb_amd.Engine(p) as eng:
+            if ck is None:
+                res = eng.solve(pool_size=4096, max_batches=10, budget=seg_budget)
+            else:
+                res = eng.resume(ck, pool_size=4096, max_batches=10, budget=seg_budget)
+            ck = eng.checkpoint()
+        if seg_budget:
+            assert not res["proven"] and ck["permT"].shape[1] > 0
+            assert ck["stats"][0] <= seg_budget
+    assert res["proven"] and res["best"] == full["best"] == 1582
+    assert ck["permT"].shape[1] == 0
+    assert cmax_np(p, res["perm"]) == res["best"]
+    for k in STATS:
+        assert res["stats"][k] == full["stats"][k], k

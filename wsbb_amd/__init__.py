@@ -70,6 +70,12 @@ def _load():
         "wsbb_search_profile": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _i64p],
                                 ctypes.c_int),
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
+        "wsbb_search_options": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_search_state": ([_vp, _i32p, _i64p, _i32p, _i32p, _u8p, _i32p, _i32p, _i32p,
+                               ctypes.c_int], ctypes.c_int),
+        "wsbb_solve_resume": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                               _u8p, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int32, _i64p,
+                               _i32p, _i32p, _i32p, _i64p], ctypes.c_int),
         "wsbb_copy_device": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
         "wsbb_selector_init": ([_vp, ctypes.c_uint64], ctypes.c_int),
         "wsbb_selector_params": ([_vp, ctypes.c_int, _f32p, _f32p, _f32p, _f32p], ctypes.c_int),
@@ -278,6 +284,11 @@ class Engine:
     def set_pool_capacity(self, nodes):
         _check(lib.wsbb_set_pool_capacity(self.h, nodes), "set_pool_capacity")
 
+    def search_options(self, use_graph=-1, rounds_per_sync=-1):
+        """hipGraph replay of the search rounds (1/0) and rounds per host
+        check; -1 keeps a setting."""
+        _check(lib.wsbb_search_options(self.h, use_graph, rounds_per_sync), "search_options")
+
     # ---- learned pair selector ----
     HIDDEN = 128
 
@@ -353,5 +364,39 @@ class Engine:
                             ctypes.byref(best), _p(bp), _p(st, _i64p))
         if rc not in (0, -5):
             _check(rc, "solve")
+        return dict(best=best.value, perm=bp, stats=dict(zip(STAT_NAMES, st.tolist())),
+                    proven=(rc == 0))
+
+    def checkpoint(self):
+        """Device state of the last solve: incumbent, counters, incumbent
+        permutation and the node pool it stopped with."""
+        inc, cnt = ctypes.c_int32(), ctypes.c_int32()
+        st = np.zeros(NSTATS, np.int64)
+        bp = np.zeros(self.n, np.int32)
+        _check(lib.wsbb_search_state(self.h, ctypes.byref(inc), _p(st, _i64p), _p(bp),
+                                     ctypes.byref(cnt), None, None, None, None, 0), "search_state")
+        c = cnt.value
+        permT = np.zeros((self.n, c), np.uint8)
+        l1, l2, cost = (np.zeros(c, np.int32) for _ in range(3))
+        _check(lib.wsbb_search_state(self.h, ctypes.byref(inc), _p(st, _i64p), _p(bp),
+                                     ctypes.byref(cnt), _p(permT, _u8p), _p(l1), _p(l2), _p(cost),
+                                     c), "search_state")
+        return dict(inc=inc.value, stats=st, best_perm=bp, permT=permT, l1=l1, l2=l2, cost=cost)
+
+    def resume(self, ckpt, kind=LB2, pool_size=4096, max_batches=1, budget=0):
+        """Continue a search from a checkpoint() dict."""
+        permT = np.ascontiguousarray(ckpt["permT"], np.uint8)
+        l1, l2, cost = (np.ascontiguousarray(ckpt[k], np.int32) for k in ("l1", "l2", "cost"))
+        st_in = np.ascontiguousarray(ckpt["stats"], np.int64)
+        bp_in = np.ascontiguousarray(ckpt["best_perm"], np.int32)
+        best = ctypes.c_int32()
+        bp = np.zeros(self.n, np.int32)
+        st = np.zeros(NSTATS, np.int64)
+        rc = lib.wsbb_solve_resume(self.h, kind, pool_size, max_batches, budget, _p(permT, _u8p),
+                                   _p(l1), _p(l2), _p(cost), l1.shape[0], int(ckpt["inc"]),
+                                   _p(st_in, _i64p), _p(bp_in), ctypes.byref(best), _p(bp),
+                                   _p(st, _i64p))
+        if rc not in (0, -5):
+            _check(rc, "resume")
         return dict(best=best.value, perm=bp, stats=dict(zip(STAT_NAMES, st.tolist())),
                     proven=(rc == 0))

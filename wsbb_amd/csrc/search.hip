@@ -284,21 +284,25 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     if (stop) break;
   }
   if (tid == 0) {
-    C->stats[kStRounds] += 1;
     if (np > S.PCAP || nc > S.CCAP) {
       C->status = kOverflow;
       C->n_parents = 0;
       C->n_children = 0;
       return;
     }
-    S.p_off[np] = nc;
-    C->stats[kStPrunedPop] += pruned;
-    C->stack_size = top - consumed;
-    if (S.budget > 0 && C->stats[kStDecomposed] + np > S.budget) {
+    if (S.budget > 0 && np > 0 && C->stats[kStDecomposed] + np > S.budget) {
+      // stop before this round: the stack keeps every candidate, so a
+      // resumed search continues exactly where this one stopped
       C->status = kBudget;
       C->n_parents = 0;
       C->n_children = 0;
-    } else if (np == 0) {
+      return;
+    }
+    C->stats[kStRounds] += 1;
+    S.p_off[np] = nc;
+    C->stats[kStPrunedPop] += pruned;
+    C->stack_size = top - consumed;
+    if (np == 0) {
       C->status = kDone;
       C->n_parents = 0;
       C->n_children = 0;
@@ -713,6 +717,8 @@ struct SearchState {
   int* r_label = nullptr;
   uint32_t* r_tsel = nullptr;
   int* r_lb1 = nullptr;
+  // rounds_per_sync rounds captured once per solve and replayed
+  hipGraphExec_t graph = nullptr;
   // optional timing of every bound launch (pairs of events per round)
   std::vector<hipEvent_t> ev;
   double bound_ms = 0;
@@ -743,6 +749,7 @@ static int ensure_aux(wsbb_ctx* ctx) {
 void destroy_search(wsbb_ctx* ctx) {
   if (!ctx->search) return;
   for (hipEvent_t e : ctx->search->ev) (void)hipEventDestroy(e);
+  if (ctx->search->graph) (void)hipGraphExecDestroy(ctx->search->graph);
   ctx->search->aux.release();
   ctx->search->mem.release();
   if (ctx->search->host_ctrl) (void)hipHostFree(ctx->search->host_ctrl);
@@ -892,7 +899,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
 
 int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
           int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
-          int cap_nodes, int rounds_per_sync) {
+          int cap_nodes, int rounds_per_sync, const ResumeState* resume) {
   const int n = ctx->T.n;
   if (pool_size - 1 <= 2 * n || max_batches < 1 || max_batches > 64 ||
       static_cast<long long>(pool_size) * max_batches > (1 << 23)) {
@@ -921,8 +928,48 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   A.max_batches = max_batches;
   A.budget = budget;
   A.count_pairs = kind == WSBB_LB2;
-  hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");
+  if (resume == nullptr) {
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");
+  } else {
+    const int cnt = resume->count;
+    if (cnt < 0 || cnt > S->CAP) {
+      set_error("saved pool larger than the device pool capacity");
+      return WSBB_EOVERFLOW;
+    }
+    for (int k = 0; k < cnt; ++k) {
+      const int a = resume->l1[k], b = resume->l2[k];
+      bool ok = a >= -1 && b <= n && a < b;
+      for (int i = 0; ok && i < n; ++i) ok = resume->permT[static_cast<size_t>(i) * cnt + k] < n;
+      if (!ok) {
+        set_error("saved pool holds a node with bad limits or job ids");
+        return WSBB_EINVAL;
+      }
+    }
+    Ctrl h{};
+    h.status = kRunning;
+    h.stack_size = cnt;
+    h.inc = ub0;
+    if (resume->stats)
+      for (int i = 0; i < WSBB_NSTATS; ++i) h.stats[i] = resume->stats[i];
+    if (resume->best_perm && resume->best_perm[0] >= 0) {
+      h.have_perm = 1;
+      for (int i = 0; i < n; ++i) h.best_perm[i] = static_cast<uint8_t>(resume->best_perm[i]);
+    }
+    e = hipMemcpyAsync(A.ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream);
+    for (int i = 0; i < n && e == hipSuccess && cnt > 0; ++i)
+      e = hipMemcpyAsync(A.s_perm + static_cast<size_t>(i) * S->CAP,
+                         resume->permT + static_cast<size_t>(i) * cnt, cnt,
+                         hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && cnt > 0)
+      e = hipMemcpyAsync(A.s_l1, resume->l1, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && cnt > 0)
+      e = hipMemcpyAsync(A.s_l2, resume->l2, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && cnt > 0)
+      e = hipMemcpyAsync(A.s_cost, resume->cost, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "upload of the saved pool");
+  }
   const bool timing = ctx->profile_bound;
   if (timing && static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
     while (static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
@@ -933,11 +980,44 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   }
   S->bound_ms = 0;
   S->bound_launches = 0;
+  // hipGraph replay: the kernels of rounds_per_sync rounds are captured once
+  // (their arguments do not change during a solve; all round state lives in
+  // device memory) and replayed until the pool state says stop
+  const bool use_graph = ctx->search_graph && !timing;
+  if (S->graph) {
+    (void)hipGraphExecDestroy(S->graph);
+    S->graph = nullptr;
+  }
+  if (use_graph) {
+    hipGraph_t g = nullptr;
+    if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(e, "hipStreamBeginCapture");
+    hipError_t le = hipSuccess;
+    for (int r = 0; r < rounds_per_sync && le == hipSuccess; ++r)
+      le = launch_round(ctx, A, kind, nullptr, nullptr);
+    e = hipStreamEndCapture(ctx->stream, &g);
+    if (le != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(le, "round capture");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&S->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      S->graph = nullptr;
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+  }
   for (;;) {
-    for (int r = 0; r < rounds_per_sync; ++r)
-      if ((e = launch_round(ctx, A, kind, timing ? S->ev[2 * r] : nullptr,
-                            timing ? S->ev[2 * r + 1] : nullptr)) != hipSuccess)
-        return hip_fail(e, "round launch");
+    if (use_graph) {
+      if ((e = hipGraphLaunch(S->graph, ctx->stream)) != hipSuccess)
+        return hip_fail(e, "hipGraphLaunch");
+    } else {
+      for (int r = 0; r < rounds_per_sync; ++r)
+        if ((e = launch_round(ctx, A, kind, timing ? S->ev[2 * r] : nullptr,
+                              timing ? S->ev[2 * r + 1] : nullptr)) != hipSuccess)
+          return hip_fail(e, "round launch");
+    }
     e = hipMemcpyAsync(S->host_ctrl, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "search sync");
@@ -991,5 +1071,79 @@ extern "C" int wsbb_solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batche
     return WSBB_EINVAL;
   }
   return wsbb::solve(ctx, kind, pool_size, max_batches, ub0, budget, best, best_perm,
-                     stats, ctx->pool_capacity, 4);
+                     stats, ctx->pool_capacity, ctx->rounds_per_sync);
+}
+
+extern "C" int wsbb_search_state(wsbb_ctx* ctx, int32_t* inc, int64_t* stats,
+                                 int32_t* best_perm, int* count, uint8_t* permT,
+                                 int32_t* l1, int32_t* l2, int32_t* cost, int cap) {
+  using namespace wsbb;
+  if (!ctx || !count) {
+    set_error("null context or count");
+    return WSBB_EINVAL;
+  }
+  if (!ctx->search) {
+    set_error("no search state: call wsbb_solve first");
+    return WSBB_EINVAL;
+  }
+  SearchState* S = ctx->search;
+  const int n = ctx->T.n;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  Ctrl h{};
+  if (e == hipSuccess) e = hipMemcpy(&h, S->args.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "search state read");
+  *count = h.stack_size;
+  if (inc) *inc = h.inc;
+  if (stats)
+    for (int i = 0; i < WSBB_NSTATS; ++i) stats[i] = h.stats[i];
+  if (best_perm)
+    for (int i = 0; i < n; ++i) best_perm[i] = h.have_perm ? h.best_perm[i] : -1;
+  if (!permT && !l1 && !l2 && !cost) return WSBB_OK;
+  if (cap < h.stack_size || !permT || !l1 || !l2 || !cost) {
+    set_error("pool buffers missing or smaller than the pool (see count)");
+    return WSBB_EINVAL;
+  }
+  const int cnt = h.stack_size;
+  for (int i = 0; i < n && e == hipSuccess && cnt > 0; ++i)
+    e = hipMemcpy(permT + static_cast<size_t>(i) * cnt, S->args.s_perm + static_cast<size_t>(i) * S->CAP,
+                  cnt, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && cnt > 0) e = hipMemcpy(l1, S->args.s_l1, 4 * size_t(cnt), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && cnt > 0) e = hipMemcpy(l2, S->args.s_l2, 4 * size_t(cnt), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && cnt > 0) e = hipMemcpy(cost, S->args.s_cost, 4 * size_t(cnt), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "search pool read");
+  return WSBB_OK;
+}
+
+extern "C" int wsbb_solve_resume(wsbb_ctx* ctx, int kind, int pool_size, int max_batches,
+                                 int64_t budget, const uint8_t* permT, const int32_t* l1,
+                                 const int32_t* l2, const int32_t* cost, int count,
+                                 int32_t inc, const int64_t* stats_in,
+                                 const int32_t* best_perm_in, int32_t* best,
+                                 int32_t* best_perm, int64_t* stats) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (count < 0 || (count > 0 && (!permT || !l1 || !l2 || !cost))) {
+    wsbb::set_error("bad saved pool");
+    return WSBB_EINVAL;
+  }
+  const wsbb::ResumeState R{permT, l1, l2, cost, count, stats_in, best_perm_in};
+  return wsbb::solve(ctx, kind, pool_size, max_batches, inc, budget, best, best_perm, stats,
+                     ctx->pool_capacity, ctx->rounds_per_sync, &R);
+}
+
+extern "C" int wsbb_search_options(wsbb_ctx* ctx, int use_graph, int rounds_per_sync) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (rounds_per_sync == 0 || rounds_per_sync > 64) {
+    wsbb::set_error("rounds_per_sync must be in 1..64 (-1 keeps the current value)");
+    return WSBB_EINVAL;
+  }
+  if (use_graph >= 0) ctx->search_graph = use_graph != 0;
+  if (rounds_per_sync > 0) ctx->rounds_per_sync = rounds_per_sync;
+  return WSBB_OK;
 }
