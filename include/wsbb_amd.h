@@ -172,7 +172,8 @@ int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
 int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
 /* Launch options of later solves: use_graph = 1 captures the kernels of
  * rounds_per_sync rounds in a hipGraph once per solve and replays it, 0
- * launches them directly (-1 keeps the setting; default 1); rounds_per_sync
+ * launches them directly (-1 keeps the setting; default 0: both measure
+ * the same, the rounds being bound by the device); rounds_per_sync
  * rounds run between host checks of the pool state (1..64, -1 keeps it;
  * default 8).  Bound-kernel timing (wsbb_search_profile) launches directly.
  * No counterpart in the reference (its host loop synchronises every pool,
@@ -219,6 +220,8 @@ int wsbb_selector_init(wsbb_ctx *ctx, uint64_t seed);
  * b1 [128], W2 [128][P], b2 [P]; NULL pointers are skipped. */
 int wsbb_selector_params(wsbb_ctx *ctx, int set, float *W1, float *b1,
                          float *W2, float *b2);
+/* topk pairs per 64-child tile; children whose subset bound lies in
+ * [incumbent - margin, incumbent) get full LB2 (margin -1: none) */
 int wsbb_selector_config(wsbb_ctx *ctx, int topk, int margin);
 /* log every stride-th child of later plain solves into a device store of
  * `capacity` records (stride 0 stops logging); clears the store */

@@ -21,6 +21,8 @@
 // (heuristiqueCmax 64-77) and the tail combine (cmaxFin 92-100, 147-148).
 #include "bound.h"
 
+#include <type_traits>
+
 namespace wsbb {
 
 namespace {
@@ -29,6 +31,13 @@ template <int NW>
 __device__ __forceinline__ void set_job_bit(uint32_t (&sm)[NW], int j) {
 #pragma unroll
   for (int w = 0; w < NW; ++w) sm[w] |= ((j >> 5) == w) ? (1u << (j & 31)) : 0u;
+}
+
+// float -> bfloat16, round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf_dev(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
 }
 
 __device__ __forceinline__ uint32_t bitsel(uint32_t mask, uint32_t a, uint32_t b) {
@@ -80,10 +89,12 @@ __device__ __forceinline__ void step4(const StepGroup<NW>& g, const uint32_t (&u
 // MODE 0: plain LB2 (optionally with the `best` early exit) and nothing
 // else, so none of the optional outputs holds scalar registers in the pair
 // loop; MODE 1: LB2 with the optional outputs, pair masks and tile subsets;
-// MODE 2: MODE 1 plus LB1 and the selector features.
+// MODE 2: MODE 1 plus LB1 and the selector features; MODE 3: MODE 0 over the
+// pairs of each tile's subset (tile_sel) only.  Modes 0 and 3 write their
+// result to the source's slot of the node.
 template <int MM, int NW, int MODE, int BLOCK, class Src>
 __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, BoundArgs A) {
-  constexpr bool LEAN = MODE == 0, EXTRA = MODE == 2;
+  constexpr bool LEAN = MODE == 0 || MODE == 3, EXTRA = MODE == 2;
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
   // rows m..MM-1 of the staged times are zero: the recurrences then run
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     int lb = 0, fx = -1, am = -1;
     const bool use_best = A.best != -1;
     const uint32_t* tsel = nullptr;
-    if constexpr (!LEAN)
+    if constexpr (!LEAN || MODE == 3)
       if (A.tile_sel != nullptr) tsel = A.tile_sel + static_cast<size_t>(tile) * A.nsw;
 
     // Pass 0 scans all pairs of [s_lo, s_hi), or the tile's learned subset;
@@ -243,6 +254,8 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     if constexpr (LEAN) {
       npairs = s_hi - s_lo;
       for (int s = s_lo; s < s_hi; ++s) {
+        if constexpr (MODE == 3)
+          if (!((ld_const_u32(tsel, s >> 5) >> (s & 31)) & 1u)) continue;
         const int2 ab = ld_const2(T.pairs, s);
         int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
         int t1 = r[ab.y];
@@ -327,9 +340,12 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
       if (leaf) res = leafv;
       else if (A.kind == 1) res = lb1;
       else res = fx >= 0 ? fx : lb;
-      A.out[node] = res;
-      if constexpr (!LEAN)
+      if constexpr (LEAN) {
+        A.out[src.slot(node)] = res;
+      } else {
+        A.out[node] = res;
         if (A.argmax != nullptr) A.argmax[node] = leaf ? -1 : am;
+      }
     }
     if (!LEAN && A.stats != nullptr && tsel != nullptr && grp == 0) {
       const int nvalid = __popcll(__ballot(valid));
@@ -343,6 +359,133 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
                   static_cast<unsigned long long>(nvalid) * static_cast<unsigned long long>(npairs));
       }
     }
+  }
+}
+
+// Selector features of every node as BF16 B-operand fragments of
+// v_mfma_f32_32x32x8_bf16.  5m features per node: per machine the head (R0
+// without prefix), the tail (Q0 without suffix), the remaining work of the
+// free jobs, head + work and work + tail, in that order of groups, divided by
+// `scale`.  Lane half*32 + c of the fragment of 32-node block b and feature
+// step t holds features 8t + 4half .. 8t + 4half + 3 of node 32b + c.  One
+// wave per 64-node tile; the remaining work is the row total minus the
+// scheduled jobs, which the head/tail recurrences visit anyway.
+template <int MM, int NW, class Src>
+__global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs A) {
+  extern __shared__ int32_t lds[];
+  const int n = T.n, m = T.m;
+  int32_t* sp = lds;  // [MM][n], rows m..MM-1 zero
+  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + ((MM * n + 3) & ~3));  // [n][256]
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int N = A.Ndev ? *A.Ndev : A.N;
+  const int ntiles = (N + 63) >> 6;
+  const int F = 5 * m;
+  const float sc = 1.0f / A.scale;
+  for (int tile0 = blockIdx.x * 4; tile0 < ntiles; tile0 += gridDim.x * 4) {
+    const int tile = __builtin_amdgcn_readfirstlane(tile0 + wave);
+    if (tile >= ntiles) continue;
+    const int node = tile * 64 + lane;
+    int l1 = -1, l2 = n, stride = 0, sw0 = -1, sw1 = -1;
+    const uint8_t* col = nullptr;
+    const bool valid = node < N && src.load(node, l1, l2, col, stride, sw0, sw1);
+    if (!valid) {
+      l1 = -1;
+      l2 = n;
+    }
+    const int d1 = l1 + 1, d2 = n - l2;
+    if (valid) {
+      for (int i0 = 0; i0 < n; i0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (i0 + u < n) pl[(i0 + u) * 256 + threadIdx.x] = static_cast<uint8_t>(v[u]);
+      }
+    }
+    const uint8_t* mine = pl + threadIdx.x;
+    int r[MM], q[MM], sc_[MM];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) r[k] = q[k] = sc_[k] = 0;
+    for (int i = 0; i < d1; ++i) {
+      const int j = perm_at(mine, 256, sw0, sw1, i);
+      r[0] += sp[j];
+      sc_[0] += sp[j];
+#pragma unroll
+      for (int k = 1; k < MM; ++k) {
+        const int pk = sp[k * n + j];
+        r[k] = max(r[k], r[k - 1]) + pk;
+        sc_[k] += pk;
+      }
+    }
+    for (int i = n - 1; i >= l2; --i) {
+      const int j = perm_at(mine, 256, sw0, sw1, i);
+#pragma unroll
+      for (int k = MM - 1; k >= 0; --k) {
+        const int pk = sp[k * n + j];
+        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk;
+        sc_[k] += pk;
+      }
+    }
+    // per machine: head, tail, remaining work
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      r[k] = d1 ? r[k] : ld_const(T.R0, k);  // tables are zero past machine m
+      q[k] = d2 ? q[k] : ld_const(T.Q0, k);
+      sc_[k] = ld_const(T.ptot, k) - sc_[k];
+    }
+    const int b = node >> 5, c = lane & 31;
+    int g = 0, k = 0;  // feature f = g m + k, uniform
+    for (int t = 0; t < A.fsteps; ++t)
+      for (int half = 0; half < 2; ++half) {
+        uint32_t w[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float x = 0.f;
+          if (g < 5) {
+            const int h = r[k], tl = q[k], rw = sc_[k];
+            const int v = g == 0 ? h : g == 1 ? tl : g == 2 ? rw : g == 3 ? h + rw : rw + tl;
+            x = valid ? static_cast<float>(v) * sc : 0.f;
+            if (++k == m) {
+              k = 0;
+              ++g;
+            }
+          }
+          w[i >> 1] |= static_cast<uint32_t>(f2bf_dev(x)) << (16 * (i & 1));
+        }
+        // lanes 0-31 and 32-63 belong to consecutive 32-node blocks
+        A.frag[(static_cast<size_t>(b) * A.fsteps + t) * 64 + half * 32 + c] = make_uint2(w[0], w[1]);
+      }
+    (void)F;
+  }
+}
+
+template <int MM, int NW, class Src>
+hipError_t launch_feat_mm(const DevTables& T, const Src& src, const FeatArgs& A, int grid_cap,
+                          hipStream_t stream) {
+  const int ntiles = (A.N + 63) / 64;
+  int grid = (ntiles + 3) / 4;
+  if (grid > grid_cap) grid = grid_cap;
+  if (grid < 1) grid = 1;
+  const size_t shmem = sizeof(int32_t) * ((MM * T.n + 3) & ~3) + static_cast<size_t>(T.n) * 256;
+  hipLaunchKernelGGL((k_features<MM, NW, Src>), dim3(grid), dim3(256), shmem, stream, T, src, A);
+  return hipGetLastError();
+}
+
+template <class Src>
+hipError_t launch_feat_src(const DevTables& T, const Src& src, const FeatArgs& A, int grid_cap,
+                           hipStream_t stream) {
+  if (A.N <= 0) return hipSuccess;
+  const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
+  switch (mm) {
+    case 8: return launch_feat_mm<8, 1, Src>(T, src, A, grid_cap, stream);
+    case 12: return launch_feat_mm<12, 1, Src>(T, src, A, grid_cap, stream);
+    case 16: return launch_feat_mm<16, 1, Src>(T, src, A, grid_cap, stream);
+    case 20: return launch_feat_mm<20, 1, Src>(T, src, A, grid_cap, stream);
+    default: return launch_feat_mm<24, 1, Src>(T, src, A, grid_cap, stream);
   }
 }
 
@@ -367,15 +510,16 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
                       int grid_cap, hipStream_t stream) {
   const int nw = T.n <= 32 ? 1 : (T.n <= 64 ? 2 : 4);
   const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
+  const bool bare = A.kind == 0 && A.argmax == nullptr && A.pairv == nullptr &&
+                   A.sel == nullptr && A.stats == nullptr && A.feat == nullptr;
   const int mode = (A.kind == 1 || A.feat != nullptr) ? 2
-                   : (A.kind == 0 && A.argmax == nullptr && A.pairv == nullptr &&
-                      A.sel == nullptr && A.tile_sel == nullptr && A.stats == nullptr)
-                       ? 0
-                       : 1;
+                   : (bare && A.tile_sel == nullptr)               ? 0
+                   : (bare && A.margin < 0)                        ? 3
+                                                                   : 1;
   // 8 or 16 waves per tile (1024-thread blocks) for small batches; the lean
   // variants for up to 16 machines fit four such waves per SIMD
   bool wide = A.groups > 4;
-  if (wide && (mode != 0 || mm > 16)) {
+  if (wide && (mode != 0 || mm > 16 || std::is_same<Src, ListSource>::value)) {
     // the general and wide-machine variants stop at four waves per tile
     BoundArgs A4 = A;
     A4.groups = 4;
@@ -385,9 +529,13 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
   if (mm == MMv && nw == NWv) {                                                         \
     if constexpr (MMv <= 16)                                                            \
       if (wide) return launch_mm_nw<MMv, NWv, 0, 1024, Src>(T, src, A, grid_cap, stream); \
-    return mode == 0   ? launch_mm_nw<MMv, NWv, 0, 256, Src>(T, src, A, grid_cap, stream) \
-           : mode == 1 ? launch_mm_nw<MMv, NWv, 1, 256, Src>(T, src, A, grid_cap, stream) \
-                       : launch_mm_nw<MMv, NWv, 2, 256, Src>(T, src, A, grid_cap, stream); \
+    if constexpr (std::is_same<Src, ListSource>::value)                                   \
+      return launch_mm_nw<MMv, NWv, 0, 256, Src>(T, src, A, grid_cap, stream);            \
+    else                                                                                \
+      return mode == 0   ? launch_mm_nw<MMv, NWv, 0, 256, Src>(T, src, A, grid_cap, stream) \
+             : mode == 3 ? launch_mm_nw<MMv, NWv, 3, 256, Src>(T, src, A, grid_cap, stream) \
+             : mode == 1 ? launch_mm_nw<MMv, NWv, 1, 256, Src>(T, src, A, grid_cap, stream) \
+                         : launch_mm_nw<MMv, NWv, 2, 256, Src>(T, src, A, grid_cap, stream); \
   }
   WSBB_DISPATCH(8, 1) WSBB_DISPATCH(8, 2) WSBB_DISPATCH(8, 4)
   WSBB_DISPATCH(12, 1) WSBB_DISPATCH(12, 2) WSBB_DISPATCH(12, 4)
@@ -400,6 +548,16 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
 
 }  // namespace
 
+hipError_t launch_features(const DevTables& T, const BatchSource& src, const FeatArgs& A,
+                           int grid_cap, hipStream_t stream) {
+  return launch_feat_src(T, src, A, grid_cap, stream);
+}
+
+hipError_t launch_features(const DevTables& T, const ChildSource& src, const FeatArgs& A,
+                           int grid_cap, hipStream_t stream) {
+  return launch_feat_src(T, src, A, grid_cap, stream);
+}
+
 hipError_t launch_bound(const DevTables& T, const BatchSource& src,
                         const BoundArgs& A, int grid_cap, hipStream_t stream) {
   if (A.N <= 0) return hipSuccess;
@@ -409,6 +567,14 @@ hipError_t launch_bound(const DevTables& T, const BatchSource& src,
 hipError_t launch_bound(const DevTables& T, const ChildSource& src,
                         const BoundArgs& A, int grid_cap, hipStream_t stream) {
   if (A.N <= 0) return hipSuccess;
+  return launch_src(T, src, A, grid_cap, stream);
+}
+
+hipError_t launch_bound(const DevTables& T, const ListSource& src,
+                        const BoundArgs& A, int grid_cap, hipStream_t stream) {
+  if (A.N <= 0) return hipSuccess;
+  if (A.kind != 0 || A.argmax || A.pairv || A.sel || A.tile_sel || A.stats || A.feat)
+    return hipErrorInvalidValue;  // listed nodes get plain LB2 only
   return launch_src(T, src, A, grid_cap, stream);
 }
 

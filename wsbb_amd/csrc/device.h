@@ -83,6 +83,7 @@ struct BatchSource {
     sw0 = sw1 = -1;
     return true;
   }
+  __device__ __forceinline__ int slot(int k) const { return k; }
 };
 
 // Node source: children described by (parent, side, position) over a
@@ -109,6 +110,21 @@ struct ChildSource {
     stride = PN;
     return true;
   }
+  __device__ __forceinline__ int slot(int k) const { return k; }
+};
+
+// Node source: the children listed in list[0..*count) (a compacted subset of
+// a round's children); results go to the listed child's slot.
+struct ListSource {
+  ChildSource base;
+  const int32_t* list;
+  const int* count;
+  __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
+                                       int& stride, int& sw0, int& sw1) const {
+    if (k >= *count) return false;
+    return base.load(list[k], a, b, col, stride, sw0, sw1);
+  }
+  __device__ __forceinline__ int slot(int k) const { return list[k]; }
 };
 
 __device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,

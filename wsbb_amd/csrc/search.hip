@@ -60,14 +60,17 @@ struct Ctrl {
   int push_base;
   int push_total;
   int have_perm;
-  int pad[8];
+  int n_fb;  // children of the round listed for the full-LB2 fallback
+  int pad[7];
   long long stats[WSBB_NSTATS];
   uint8_t best_perm[kMaxJobs];
 };
 
 struct SArgs {
   int n, m, P, CAP, PCAP, CCAP, pool_m1, max_batches;
-  int count_pairs;  // every child evaluates all pairs (plain LB2)
+  int count_pairs;  // 1: every child evaluates all P pairs (plain LB2);
+                    // 2: topk pairs per child (learned subsets)
+  int topk;
   long long budget;
   Ctrl* ctrl;
   uint8_t* s_perm;  // [n][CAP]
@@ -311,7 +314,12 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
       C->n_children = nc;
       C->stats[kStDecomposed] += np;
       C->stats[kStBounded] += nc;
-      if (S.count_pairs) C->stats[kStPairs] += static_cast<long long>(nc) * S.P;
+      if (S.count_pairs == 1) C->stats[kStPairs] += static_cast<long long>(nc) * S.P;
+      if (S.count_pairs == 2) {
+        C->stats[kStPairs] += static_cast<long long>(nc) * S.topk;
+        C->stats[kStSubset] += nc;
+      }
+      C->n_fb = 0;
     }
   }
 }
@@ -651,6 +659,38 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
   }
 }
 
+// Children whose learned-subset bound lies in [incumbent - margin,
+// incumbent) and that are not leaves are listed for full LB2 (the order of
+// the list does not matter: every listed child gets its own exact value).
+__global__ __launch_bounds__(256) void k_fallback_list(SArgs S, int margin, int* list) {
+  Ctrl* C = S.ctrl;
+  const int nc = C->n_children, inc = C->inc, n = S.n;
+  const int lane = threadIdx.x & 63;
+  for (int c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += gridDim.x * blockDim.x) {
+    const int c = c0 + threadIdx.x;
+    bool take = false;
+    if (c < nc) {
+      const int b = S.c_bound[c];
+      if (b < inc && b >= inc - margin) {
+        const int p = S.c_desc[c] >> 9;
+        take = S.p_l2[p] - S.p_l1[p] - 1 > 2;  // parents with u <= 2 have leaf children
+      }
+    }
+    const unsigned long long bal = __ballot(take);
+    if (bal == 0) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&C->n_fb, __popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (take) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = c;
+  }
+}
+
+__global__ void k_fallback_stats(SArgs S) {
+  Ctrl* C = S.ctrl;
+  C->stats[kStFallback] += C->n_fb;
+  C->stats[kStPairs] += static_cast<long long>(C->n_fb) * S.P;
+}
+
 __global__ void k_init(SArgs S, int ub0) {
   Ctrl* C = S.ctrl;
   const int n = S.n;
@@ -711,12 +751,14 @@ struct SearchState {
   DevBuf mem;
   SArgs args{};
   Ctrl* host_ctrl = nullptr;  // pinned
-  // per-round selector buffers: features [5m][CCAP], labels, tile masks, LB1
+  // per-round selector buffers: logged features [5m][CCAP] and labels, BF16
+  // feature fragments, tile masks, the fallback list
   DevBuf aux;
   float* r_feat = nullptr;
   int* r_label = nullptr;
   uint32_t* r_tsel = nullptr;
-  int* r_lb1 = nullptr;
+  uint2* r_xfrag = nullptr;
+  int* r_fbl = nullptr;
   // rounds_per_sync rounds captured once per solve and replayed
   hipGraphExec_t graph = nullptr;
   // optional timing of every bound launch (pairs of events per round)
@@ -728,21 +770,23 @@ struct SearchState {
 static int ensure_aux(wsbb_ctx* ctx) {
   SearchState* S = ctx->search;
   if (S->r_feat) return WSBB_OK;
-  const int F = 5 * ctx->T.m, nsw = (ctx->T.P + 31) / 32;
+  const int F = 5 * ctx->T.m, nsw = (ctx->T.P + 31) / 32, fsteps = (F + 7) / 8;
   const size_t ntiles = (static_cast<size_t>(S->CCAP) + 63) / 64;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t bf = al(sizeof(float) * F * static_cast<size_t>(S->CCAP));
   const size_t bl = al(sizeof(int) * static_cast<size_t>(S->CCAP));
   const size_t bt = al(sizeof(uint32_t) * ntiles * nsw);
-  hipError_t e = S->aux.reserve(bf + 2 * bl + bt);
+  const size_t bx = al(sizeof(uint2) * 2 * ntiles * fsteps * 64);
+  hipError_t e = S->aux.reserve(bf + 2 * bl + bt + bx);
   if (e != hipSuccess) {
     hip_fail(e, "hipMalloc(selector round buffers)");
     return WSBB_ENOMEM;
   }
   S->r_feat = S->aux.as<float>(0);
   S->r_label = S->aux.as<int>(bf);
-  S->r_lb1 = S->aux.as<int>(bf + bl);
+  S->r_fbl = S->aux.as<int>(bf + bl);
   S->r_tsel = S->aux.as<uint32_t>(bf + 2 * bl);
+  S->r_xfrag = S->aux.as<uint2>(bf + 2 * bl + bt);
   return WSBB_OK;
 }
 
@@ -850,24 +894,18 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
   hipError_t e;
   if (kind == WSBB_LEARNED) {
-    // features of every child, MLP pair subset per 64-child tile, then LB2
-    // over the subset with the full-LB2 fallback near the incumbent
-    BoundArgs Fa = B;
-    Fa.kind = 1;
-    Fa.groups = 1;
-    Fa.out = SS->r_lb1;
-    Fa.feat = SS->r_feat;
-    Fa.ld = A.CCAP;
-    if ((e = launch_bound(ctx->T, src, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
-    if ((e = launch_mlp_select(ctx, SS->r_feat, A.CCAP, A.CCAP, &A.ctrl->n_children, SS->r_tsel,
-                               nullptr)) != hipSuccess)
+    // BF16 features of every child, MLP pair subset per 64-child tile, LB2
+    // over the subset; children close to the incumbent then get full LB2
+    FeatArgs Fa{A.CCAP, &A.ctrl->n_children, static_cast<float>(ctx->inst.path_bound),
+                selector_fsteps(ctx), SS->r_xfrag};
+    if ((e = launch_features(ctx->T, src, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+    if ((e = launch_mlp_select(ctx, SS->r_xfrag, A.CCAP, &A.ctrl->n_children, SS->r_tsel,
+                               nullptr, 0)) != hipSuccess)
       return e;
     B.groups = 1;
     B.tile_sel = SS->r_tsel;
     B.nsw = selector_nsw(ctx);
-    B.margin = selector_margin(ctx);
-    B.inc_dev = &A.ctrl->inc;
-    B.stats = &A.ctrl->stats[kStSubset];
+    B.margin = -1;
   } else if (log_stride > 0) {
     B.feat = SS->r_feat;
     B.argmax = SS->r_label;
@@ -876,6 +914,21 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   if (ev0) (void)hipEventRecord(ev0, st);
   e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
   if (e != hipSuccess) return e;
+  if (kind == WSBB_LEARNED && selector_margin(ctx) >= 0) {
+    hipLaunchKernelGGL(k_fallback_list, dim3((A.CCAP + 255) / 256), dim3(256), 0, st, A,
+                       selector_margin(ctx), SS->r_fbl);
+    ListSource ls{src, SS->r_fbl, &A.ctrl->n_fb};
+    BoundArgs Bf{};
+    Bf.N = A.CCAP;
+    Bf.Ndev = &A.ctrl->n_fb;
+    Bf.kind = 0;
+    Bf.best = -1;
+    Bf.groups = B.groups > 1 ? B.groups : 4;
+    Bf.margin = -1;
+    Bf.out = A.c_bound;
+    if ((e = launch_bound(ctx->T, ls, Bf, ctx->grid_cap(), st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fallback_stats, dim3(1), dim3(1), 0, st, A);
+  }
   if (ev1) (void)hipEventRecord(ev1, st);
   if (kind != WSBB_LEARNED && log_stride > 0) {
     LogArgs L{};
@@ -927,7 +980,8 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   A.pool_m1 = pool_size - 1;
   A.max_batches = max_batches;
   A.budget = budget;
-  A.count_pairs = kind == WSBB_LB2;
+  A.count_pairs = kind == WSBB_LB2 ? 1 : kind == WSBB_LEARNED ? 2 : 0;
+  A.topk = kind == WSBB_LEARNED ? std::min(selector_topk(ctx), ctx->T.P) : 0;
   if (resume == nullptr) {
     hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");

@@ -8,10 +8,13 @@ struct wsbb_ctx;
 
 namespace wsbb {
 
-// MLP scores of N nodes (features [5m][ld]) -> per-64-node-tile pair masks
-// [ntiles][ceil(P/32)] holding the tile's top-k pairs; optional logits.
-hipError_t launch_mlp_select(wsbb_ctx* ctx, const float* feat, int ld, int N, const int* Ndev,
-                             uint32_t* tile_sel, float* scores);
+// MLP scores of N nodes (features as BF16 B fragments
+// [2 ceil(N/64)][ceil(5m/8)][64], see FeatArgs) -> per-64-node-tile pair
+// masks [ntiles][ceil(P/32)] holding the tile's top-k pairs; optional logits
+// [P][ld].
+hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int* Ndev,
+                             uint32_t* tile_sel, float* scores, int ld);
+int selector_fsteps(const wsbb_ctx* ctx);
 int selector_topk(const wsbb_ctx* ctx);
 int selector_margin(const wsbb_ctx* ctx);
 bool selector_ready(const wsbb_ctx* ctx);

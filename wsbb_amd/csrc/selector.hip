@@ -52,57 +52,87 @@ __host__ __device__ inline float bf2f(uint16_t h) {
 __host__ __device__ inline float bfround(float f) { return bf2f(f2bf(f)); }
 
 // ---------------------------------------------------------------- inference
+constexpr int kMaxFsteps = 15;  // ceil(5m / 8) for m <= 24
+
 struct MlpArgs {
-  int N, F, Fsteps, P, nsw, topk;
+  int N, P, nsw, topk, fsteps;
   const int* Ndev;          // device node count, or null
-  const float* feat;        // [F][ld]
-  int ld;
-  const uint2* frag1;       // [4][Fsteps][64] A fragments of W1^T
-  const uint2* frag2;       // [PB][16][64]   A fragments of W2^T
+  const uint2* xfrag;       // [2 ntiles][fsteps][64] feature B fragments
+  const uint2* frag1;       // [4][fsteps][64] A fragments of W1^T
+  const uint2* frag2;       // [PB][16][64]    A fragments of W2^T
   const float* b1;          // [128]
   const float* b2;          // [PB*32], padding = -inf
   uint32_t* tile_sel;       // [ntiles][nsw]
   float* scores;            // [P][ld] logits, or null
+  int ld;
 };
 
+__device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
+  bf16x4 r;
+  r[0] = static_cast<short>(a.x & 0xffff);
+  r[1] = static_cast<short>(a.x >> 16);
+  r[2] = static_cast<short>(a.y & 0xffff);
+  r[3] = static_cast<short>(a.y >> 16);
+  return r;
+}
+
+// Four waves per block, one 64-node tile per wave and iteration; the weight
+// fragments of both layers are staged in LDS once per block.
 template <int PB>
-__global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
-  __shared__ float s_p[PB * 32][33];  // probabilities of one 32-node block
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
+  extern __shared__ uint2 s_w[];
+  const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
+  // 16 loads in flight per thread before their LDS stores
+  for (int i0 = 0; i0 < n1 + n2; i0 += 16 * 256) {
+    uint2 v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int i = i0 + u * 256 + threadIdx.x;
+      if (i < n1 + n2) v[u] = i < n1 ? A.frag1[i] : A.frag2[i - n1];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int i = i0 + u * 256 + threadIdx.x;
+      if (i < n1 + n2) s_w[i] = v[u];
+    }
+  }
+  __syncthreads();
+  const uint2* w1 = s_w;
+  const uint2* w2 = s_w + n1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* s_p = reinterpret_cast<float*>(s_w + n1 + n2) + wave * 32 * 33;  // [32 pairs][33]
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
   const int col = lane & 31, half = lane >> 5;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    float agg[PB / 2];  // pair sums, pairs lane + 64*i
+  for (int tile0 = blockIdx.x * 4; tile0 < ntiles; tile0 += gridDim.x * 4) {
+    const int tile = __builtin_amdgcn_readfirstlane(tile0 + wave);
+    if (tile >= ntiles) continue;  // no block barrier below
+    float agg[PB];  // tile sums of the probabilities of pairs 32 bp + col
 #pragma unroll
-    for (int i = 0; i < PB / 2; ++i) agg[i] = 0.f;
+    for (int bp = 0; bp < PB; ++bp) agg[bp] = 0.f;
     for (int nb = 0; nb < 2; ++nb) {
-      const int node = tile * 64 + nb * 32 + col;
+      const int blk = tile * 2 + nb;
+      const int node = blk * 32 + col;
       const bool valid = node < N;
-      // layer 1: H^T[128 x 32] = W1^T[128 x Fp] . X^T[Fp x 32]
+      // layer 1: H^T[128 x 32] = W1^T[128 x F] . X^T[F x 32]; the next
+      // feature fragment is loaded while this step multiplies
+      const uint2* xp = A.xfrag + static_cast<size_t>(blk) * A.fsteps * 64 + lane;
+      // the accumulators start from the biases: row 32bi + 8q + 4half + r of
+      // the lane sits in acc1[bi][4q + r]
       f32x16 acc1[4];
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) acc1[bi][v] = 0.f;
-      for (int t = 0; t < A.Fsteps; ++t) {
-        bf16x4 b;
+        for (int v = 0; v < 16; ++v) acc1[bi][v] = A.b1[32 * bi + 8 * (v >> 2) + 4 * half + (v & 3)];
+      uint2 xn = xp[0];
+#pragma unroll 1
+      for (int t = 0; t < A.fsteps; ++t) {
+        const bf16x4 b = as_bf16x4(xn);
+        if (t + 1 < A.fsteps) xn = xp[(t + 1) * 64];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int f = 8 * t + 4 * half + i;
-          const float x = (valid && f < A.F) ? A.feat[static_cast<size_t>(f) * A.ld + node] : 0.f;
-          b[i] = static_cast<short>(f2bf(x));
-        }
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-          const uint2 a2 = A.frag1[(bi * A.Fsteps + t) * 64 + lane];
-          bf16x4 a;
-          a[0] = static_cast<short>(a2.x & 0xffff);
-          a[1] = static_cast<short>(a2.x >> 16);
-          a[2] = static_cast<short>(a2.y & 0xffff);
-          a[3] = static_cast<short>(a2.y >> 16);
-          acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc1[bi], 0, 0, 0);
-        }
+        for (int bi = 0; bi < 4; ++bi)
+          acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+              as_bf16x4(w1[(bi * A.fsteps + t) * 64 + lane]), b, acc1[bi], 0, 0, 0);
       }
       // bias, ReLU, BF16: lane holds hidden 32bi + 8q + 4half + r for its node,
       // which is the layer-2 B operand of k-step 4bi + q
@@ -114,7 +144,7 @@ __global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int h = 32 * bi + 8 * q + 4 * half + r;
-            const float y = fmaxf(acc1[bi][4 * q + r] + A.b1[h], 0.f);
+            const float y = fmaxf(acc1[bi][4 * q + r], 0.f);
             hb[4 * bi + q][r] = static_cast<short>(f2bf(y));
           }
       // layer 2: Z^T[PB*32 x 32] = W2^T . H^T
@@ -122,31 +152,43 @@ __global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
 #pragma unroll
       for (int bp = 0; bp < PB; ++bp)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) acc2[bp][v] = 0.f;
+        for (int v = 0; v < 16; ++v) acc2[bp][v] = A.b2[32 * bp + 8 * (v >> 2) + 4 * half + (v & 3)];
+      // fully unrolled (hb is indexed statically); the next step's weight
+      // fragments are read while this step multiplies, and the scheduling
+      // barrier keeps the compiler from hoisting further reads
+      uint2 wa[PB], wb[PB];
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
+      for (int bp = 0; bp < PB; ++bp) wa[bp] = w2[(bp * 16) * 64 + lane];
 #pragma unroll
-        for (int bp = 0; bp < PB; ++bp) {
-          const uint2 a2 = A.frag2[(bp * 16 + t) * 64 + lane];
-          bf16x4 a;
-          a[0] = static_cast<short>(a2.x & 0xffff);
-          a[1] = static_cast<short>(a2.x >> 16);
-          a[2] = static_cast<short>(a2.y & 0xffff);
-          a[3] = static_cast<short>(a2.y >> 16);
-          acc2[bp] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[t], acc2[bp], 0, 0, 0);
+      for (int t = 0; t < 16; ++t) {
+        if (t + 1 < 16) {
+#pragma unroll
+          for (int bp = 0; bp < PB; ++bp) wb[bp] = w2[(bp * 16 + t + 1) * 64 + lane];
         }
+#pragma unroll
+        for (int bp = 0; bp < PB; ++bp)
+          acc2[bp] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_bf16x4(wa[bp]), hb[t], acc2[bp], 0, 0, 0);
+#pragma unroll
+        for (int bp = 0; bp < PB; ++bp) wa[bp] = wb[bp];
+        __builtin_amdgcn_sched_barrier(0);
+      }
       // logits; this lane's pairs are 32bp + 8(v/4) + 4half + v%4
+      if (A.scores != nullptr) {
+        int P = A.P;
+        asm volatile("" : "+s"(P));  // keep the bounds tests out of the loop preheader
+#pragma unroll
+        for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int s = 32 * bp + 8 * (v >> 2) + 4 * half + (v & 3);
+            if (valid && s < P) A.scores[static_cast<size_t>(s) * A.ld + node] = acc2[bp][v];
+          }
+      }
       float mx = -INFINITY;
 #pragma unroll
       for (int bp = 0; bp < PB; ++bp)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int s = 32 * bp + 8 * (v >> 2) + 4 * half + (v & 3);
-          const float z = acc2[bp][v] + A.b2[s];
-          acc2[bp][v] = z;
-          if (A.scores != nullptr && valid && s < A.P) A.scores[static_cast<size_t>(s) * A.ld + node] = z;
-          mx = fmaxf(mx, z);
-        }
+        for (int v = 0; v < 16; ++v) mx = fmaxf(mx, acc2[bp][v]);
       mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
       float sum = 0.f;
 #pragma unroll
@@ -159,39 +201,39 @@ __global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
         }
       sum += __shfl_xor(sum, 32, 64);
       const float inv = valid ? 1.f / sum : 0.f;
+      // tile sums, 32 pairs at a time through this wave's LDS rows (the
+      // wave's LDS operations complete in order)
 #pragma unroll
-      for (int bp = 0; bp < PB; ++bp)
+      for (int bp = 0; bp < PB; ++bp) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int s = 32 * bp + 8 * (v >> 2) + 4 * half + (v & 3);
-          s_p[s][col] = acc2[bp][v] * inv;
-        }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < PB / 2; ++i) {
-        const int s = lane + 64 * i;
+        for (int v = 0; v < 16; ++v)
+          s_p[(8 * (v >> 2) + 4 * half + (v & 3)) * 33 + col] = acc2[bp][v] * inv;
+        __builtin_amdgcn_wave_barrier();
         float a = 0.f;
-        for (int c = 0; c < 32; ++c) a += s_p[s][c];
-        agg[i] += a;
-      }
-      __syncthreads();
-    }
-    // top-k pairs of the tile (ties: lower pair index), padding excluded
-    uint32_t mine[PB / 2];
 #pragma unroll
-    for (int i = 0; i < PB / 2; ++i) {
-      mine[i] = 0;
-      if (lane + 64 * i >= A.P) agg[i] = -1.f;
+        for (int c = 0; c < 16; ++c) a += s_p[col * 33 + half * 16 + c];
+        a += __shfl_xor(a, 32, 64);
+        agg[bp] += a;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    // top-k pairs of the tile (ties: lower pair index); lanes 0-31 hold the
+    // candidates 32 bp + col, padding and the upper half excluded
+    uint32_t mine[PB];
+#pragma unroll
+    for (int bp = 0; bp < PB; ++bp) {
+      mine[bp] = 0;
+      if (half == 1 || 32 * bp + col >= A.P) agg[bp] = -1.f;
     }
     const int k = min(A.topk, A.P);
     for (int it = 0; it < k; ++it) {
       float bv = -2.f;
       int bs = 1 << 30;
 #pragma unroll
-      for (int i = 0; i < PB / 2; ++i) {
-        const int s = lane + 64 * i;
-        if (!mine[i] && (agg[i] > bv || (agg[i] == bv && s < bs))) {
-          bv = agg[i];
+      for (int bp = 0; bp < PB; ++bp) {
+        const int s = 32 * bp + col;
+        if (!mine[bp] && (agg[bp] > bv || (agg[bp] == bv && s < bs))) {
+          bv = agg[bp];
           bs = s;
         }
       }
@@ -204,21 +246,37 @@ __global__ __launch_bounds__(64) void k_mlp_select(MlpArgs A) {
         }
       }
 #pragma unroll
-      for (int i = 0; i < PB / 2; ++i)
-        if (lane + 64 * i == bs) mine[i] = 1;
+      for (int bp = 0; bp < PB; ++bp)
+        if (half == 0 && 32 * bp + col == bs) mine[bp] = 1;
     }
-    // pack the mask: pair lane + 64i sits in word 2i (lanes 0-31) or 2i+1
-    // (lanes 32-63), bit lane & 31
+    // word bp of the mask: pairs 32 bp .. 32 bp + 31 from lanes 0-31
 #pragma unroll
-    for (int i = 0; i < PB / 2; ++i) {
-      const unsigned long long bal = __ballot(mine[i] != 0);
-      if (lane == 0) {
-        if (2 * i < A.nsw)
-          A.tile_sel[static_cast<size_t>(tile) * A.nsw + 2 * i] = static_cast<uint32_t>(bal);
-        if (2 * i + 1 < A.nsw)
-          A.tile_sel[static_cast<size_t>(tile) * A.nsw + 2 * i + 1] = static_cast<uint32_t>(bal >> 32);
-      }
+    for (int bp = 0; bp < PB; ++bp) {
+      const unsigned long long bal = __ballot(mine[bp] != 0);
+      if (lane == 0 && bp < A.nsw)
+        A.tile_sel[static_cast<size_t>(tile) * A.nsw + bp] = static_cast<uint32_t>(bal);
     }
+  }
+}
+
+// float features [F][ld] -> B fragments [2 ntiles][fsteps][64]
+__global__ void k_feat_pack(const float* feat, int ld, int N, int F, int fsteps, uint2* xfrag) {
+  const int nblk = 2 * ((N + 63) / 64);
+  const long long total = static_cast<long long>(nblk) * fsteps * 64;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int lane = static_cast<int>(i & 63);
+    const long long bt = i >> 6;
+    const int t = static_cast<int>(bt % fsteps);
+    const int blk = static_cast<int>(bt / fsteps);
+    const int node = blk * 32 + (lane & 31);
+    uint16_t v[4];
+    for (int r = 0; r < 4; ++r) {
+      const int f = 8 * t + 4 * (lane >> 5) + r;
+      v[r] = (node < N && f < F) ? f2bf(feat[static_cast<size_t>(f) * ld + node]) : 0;
+    }
+    xfrag[i] = make_uint2(v[0] | (static_cast<uint32_t>(v[1]) << 16),
+                          v[2] | (static_cast<uint32_t>(v[3]) << 16));
   }
 }
 
@@ -402,7 +460,7 @@ struct SelectorState {
   DevBuf scratch;
   int scratch_B = 0;
   // inference scratch for host-buffer calls
-  DevBuf io;
+  DevBuf io, xio;
 };
 
 void destroy_selector(wsbb_ctx* ctx) {
@@ -413,6 +471,7 @@ void destroy_selector(wsbb_ctx* ctx) {
   S->log.release();
   S->scratch.release();
   S->io.release();
+  S->xio.release();
   delete S;
   ctx->selector = nullptr;
 }
@@ -428,8 +487,7 @@ static int ensure_selector(wsbb_ctx* ctx) {
   S->F = 5 * ctx->T.m;
   S->Fsteps = (S->F + 7) / 8;
   S->P = ctx->T.P;
-  S->PB = (S->P + 31) / 32;
-  if (S->PB % 2) S->PB += 1;  // pairs are summed in lane + 64 i strips
+  S->PB = (S->P + 31) / 32;  // blocks of 32 pair scores
   S->nsw = (S->P + 31) / 32;
   const int np = param_count(*S);
   const size_t per = (static_cast<size_t>(np) * 4 + 255) & ~size_t(255);
@@ -474,38 +532,45 @@ static hipError_t refresh_frags(wsbb_ctx* ctx) {
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_select(wsbb_ctx* ctx, const float* feat, int ld, int N, const int* Ndev,
-                             uint32_t* tile_sel, float* scores) {
+hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int* Ndev,
+                             uint32_t* tile_sel, float* scores, int ld) {
   SelectorState* S = ctx->selector;
   hipError_t e = refresh_frags(ctx);
   if (e != hipSuccess) return e;
   MlpArgs A{};
   A.N = N;
   A.Ndev = Ndev;
-  A.F = S->F;
-  A.Fsteps = S->Fsteps;
   A.P = S->P;
   A.nsw = S->nsw;
   A.topk = S->topk;
-  A.feat = feat;
-  A.ld = ld;
+  A.fsteps = S->Fsteps;
+  A.xfrag = xfrag;
   A.frag1 = S->frag1;
   A.frag2 = S->frag2;
   A.b1 = S->b1;
   A.b2 = S->b2pad;
   A.tile_sel = tile_sel;
   A.scores = scores;
+  A.ld = ld;
   const int ntiles = (N + 63) / 64;
-  const int grid = std::max(1, std::min(ntiles, ctx->num_cus * 16));
+  // one persistent block per CU: the weights are staged once per block
+  const int grid = std::max(1, std::min((ntiles + 3) / 4, ctx->num_cus));
+  const size_t shmem = sizeof(uint2) * (4 * S->Fsteps * 64 + S->PB * 16 * 64) +
+                       sizeof(float) * 4 * 32 * 33;
   switch (S->PB) {
-#define WSBB_PB(v) \
-  case v: hipLaunchKernelGGL(k_mlp_select<v>, dim3(grid), dim3(64), 0, ctx->stream, A); break;
-    WSBB_PB(2) WSBB_PB(4) WSBB_PB(6) WSBB_PB(8) WSBB_PB(10)
+#define WSBB_PB(v)                                                                       \
+  case v:                                                                                \
+    hipLaunchKernelGGL(k_mlp_select<v>, dim3(grid), dim3(256), shmem, ctx->stream, A); \
+    break;
+    WSBB_PB(1) WSBB_PB(2) WSBB_PB(3) WSBB_PB(4) WSBB_PB(5) WSBB_PB(6) WSBB_PB(7) WSBB_PB(8)
+    WSBB_PB(9)
 #undef WSBB_PB
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
+
+int selector_fsteps(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->Fsteps : 0; }
 
 int selector_topk(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->topk : 0; }
 int selector_margin(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->margin : 0; }
@@ -647,8 +712,8 @@ int wsbb_selector_params(wsbb_ctx* ctx, int set, float* W1, float* b1, float* W2
 
 int wsbb_selector_config(wsbb_ctx* ctx, int topk, int margin) {
   SEL_CHECK(ctx);
-  if (topk < 1 || margin < 0) {
-    set_error("topk must be >= 1 and margin >= 0");
+  if (topk < 1 || margin < -1) {
+    set_error("topk must be >= 1 and margin >= -1 (-1: no full-LB2 fallback)");
     return WSBB_EINVAL;
   }
   ctx->selector->topk = topk;
@@ -819,7 +884,13 @@ int wsbb_selector_infer(wsbb_ctx* ctx, const float* feat, int N, float* scores, 
   float* dsc = S->io.as<float>((fb + 255) & ~size_t(255));
   uint32_t* dts = S->io.as<uint32_t>(((fb + 255) & ~size_t(255)) + ((sb + 255) & ~size_t(255)));
   SEL_HIP(hipMemcpyAsync(dfeat, feat, fb, hipMemcpyHostToDevice, ctx->stream), "H2D feat");
-  SEL_HIP(launch_mlp_select(ctx, dfeat, N, N, nullptr, dts, scores ? dsc : nullptr), "mlp launch");
+  const size_t xb = sizeof(uint2) * static_cast<size_t>(2 * ntiles) * S->Fsteps * 64;
+  SEL_HIP(S->xio.reserve(xb), "hipMalloc(feature fragments)");
+  uint2* dx = S->xio.as<uint2>();
+  hipLaunchKernelGGL(k_feat_pack, dim3(std::min<long long>((xb / 8 + 255) / 256, 4096)), dim3(256), 0,
+                     ctx->stream, dfeat, N, N, S->F, S->Fsteps, dx);
+  SEL_HIP(hipGetLastError(), "pack launch");
+  SEL_HIP(launch_mlp_select(ctx, dx, N, nullptr, dts, scores ? dsc : nullptr, N), "mlp launch");
   if (scores) SEL_HIP(hipMemcpyAsync(scores, dsc, sb, hipMemcpyDeviceToHost, ctx->stream), "D2H scores");
   if (tile_sel) SEL_HIP(hipMemcpyAsync(tile_sel, dts, tb, hipMemcpyDeviceToHost, ctx->stream), "D2H sel");
   SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
