@@ -242,6 +242,37 @@ int wsbb_selector_train(wsbb_ctx *ctx, int epochs, int batch, float lr,
 int wsbb_selector_infer(wsbb_ctx *ctx, const float *feat, int N,
                         float *scores, uint32_t *tile_sel);
 
+/* ---- mixed batches of several instances -------------------------------------
+ * No counterpart in the reference (one instance per process there).  A
+ * multi-instance handle owns one context per instance (reachable with
+ * wsbb_multi_context, e.g. to train or load its pair selector).  A batch
+ * holds the nodes of instance i at [offsets[i], offsets[i+1]); permT is
+ * [max n][N] position-major (rows past an instance's n are not read).
+ * wsbb_multi_load uploads and checks a batch once; wsbb_multi_run bounds it
+ * `reps` times, each time in one bound launch over 64-node tiles that never
+ * mix instances:
+ *   kind WSBB_LB2:     every node's bornes_calculer value for its instance;
+ *   kind WSBB_LEARNED: LB2 over each tile's top-k pairs chosen by the MLP of
+ *                      the first instance with the tile's machine count (one
+ *                      MLP launch per machine count), or over the masks in
+ *                      tile_sel [ntiles][ceil(Pmax/32)] if given (tiles in
+ *                      instance order, 64 nodes each); with inc [count] and
+ *                      margin >= 0, nodes whose subset bound lies in
+ *                      [inc - margin, inc) of their instance get full LB2.
+ * Outputs: out [N] (may be NULL), stats [3] = nodes, full-LB2 fallbacks,
+ * node-pair evaluations over all reps (may be NULL), ms = device time per
+ * rep (may be NULL). */
+typedef struct wsbb_multi wsbb_multi;
+int wsbb_multi_create(int count, const int32_t *const *p, const int *n,
+                      const int *m, int device, wsbb_multi **out);
+int wsbb_multi_destroy(wsbb_multi *multi);
+int wsbb_multi_context(wsbb_multi *multi, int i, wsbb_ctx **out);
+int wsbb_multi_load(wsbb_multi *multi, const uint8_t *permT, const int32_t *l1,
+                    const int32_t *l2, const int32_t *offsets, int N);
+int wsbb_multi_run(wsbb_multi *multi, int kind, const int32_t *inc, int margin,
+                   const uint32_t *tile_sel, int reps, int32_t *out,
+                   int64_t *stats, float *ms);
+
 #ifdef __cplusplus
 }
 #endif

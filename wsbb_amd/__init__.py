@@ -71,6 +71,13 @@ def _load():
                                 ctypes.c_int),
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_search_options": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_multi_create": ([ctypes.c_int, ctypes.POINTER(_vp), _i32p, _i32p, ctypes.c_int,
+                               ctypes.POINTER(_vp)], ctypes.c_int),
+        "wsbb_multi_destroy": ([_vp], ctypes.c_int),
+        "wsbb_multi_context": ([_vp, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "wsbb_multi_load": ([_vp, _u8p, _i32p, _i32p, _i32p, ctypes.c_int], ctypes.c_int),
+        "wsbb_multi_run": ([_vp, ctypes.c_int, _i32p, ctypes.c_int, _u32p, ctypes.c_int, _i32p,
+                            _i64p, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "wsbb_search_state": ([_vp, _i32p, _i64p, _i32p, _i32p, _u8p, _i32p, _i32p, _i32p,
                                ctypes.c_int], ctypes.c_int),
         "wsbb_solve_resume": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
@@ -159,11 +166,15 @@ def precompute(p):
 class Engine:
     """One device context: an instance's tables in HBM plus a HIP stream."""
 
-    def __init__(self, p, device=0):
+    def __init__(self, p, device=0, _handle=None):
         self.p = np.ascontiguousarray(p, np.int32)
         self.m, self.n = self.p.shape
-        h = _vp()
-        _check(lib.wsbb_create(_p(self.p), self.n, self.m, device, ctypes.byref(h)), "create")
+        self._owned = _handle is None
+        if _handle is None:
+            h = _vp()
+            _check(lib.wsbb_create(_p(self.p), self.n, self.m, device, ctypes.byref(h)), "create")
+        else:
+            h = _handle
         self.h = h
         P, cus = ctypes.c_int32(), ctypes.c_int32()
         lib.wsbb_info(self.h, None, None, ctypes.byref(P), ctypes.byref(cus))
@@ -171,7 +182,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "h", None):
-            lib.wsbb_destroy(self.h)
+            if self._owned:
+                lib.wsbb_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -400,3 +412,70 @@ class Engine:
             _check(rc, "resume")
         return dict(best=best.value, perm=bp, stats=dict(zip(STAT_NAMES, st.tolist())),
                     proven=(rc == 0))
+
+
+class Multi:
+    """Several instances whose nodes are bounded together, one launch per
+    batch (wsbb_multi_*).  `instances` is a list of p[m, n] arrays."""
+
+    def __init__(self, instances, device=0):
+        self.ps = [np.ascontiguousarray(p, np.int32) for p in instances]
+        k = len(self.ps)
+        ptrs = (_vp * k)(*[ctypes.cast(p.ctypes.data, _vp) for p in self.ps])
+        ns = np.array([p.shape[1] for p in self.ps], np.int32)
+        ms = np.array([p.shape[0] for p in self.ps], np.int32)
+        h = _vp()
+        _check(lib.wsbb_multi_create(k, ptrs, _p(ns), _p(ms), device, ctypes.byref(h)),
+               "multi_create")
+        self.h = h
+        self.maxn = int(ns.max())
+        self.engines = []
+        for i in range(k):
+            c = _vp()
+            _check(lib.wsbb_multi_context(self.h, i, ctypes.byref(c)), "multi_context")
+            self.engines.append(Engine(self.ps[i], _handle=c))
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            for e in self.engines:
+                e.close()
+            lib.wsbb_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load(self, permT, l1, l2, offsets):
+        """permT [max n][N] (rows past an instance's n unused), limits [N],
+        offsets [count + 1]: instance i owns nodes offsets[i]..offsets[i+1]-1."""
+        permT = np.ascontiguousarray(permT, np.uint8)
+        l1 = np.ascontiguousarray(l1, np.int32)
+        l2 = np.ascontiguousarray(l2, np.int32)
+        offsets = np.ascontiguousarray(offsets, np.int32)
+        self.N = l1.shape[0]
+        assert permT.shape == (self.maxn, self.N)
+        _check(lib.wsbb_multi_load(self.h, _p(permT, _u8p), _p(l1), _p(l2), _p(offsets), self.N),
+               "multi_load")
+
+    def run(self, kind=LB2, inc=None, margin=-1, tile_sel=None, reps=1):
+        out = np.zeros(self.N, np.int32)
+        st = np.zeros(3, np.int64)
+        ms = ctypes.c_float()
+        incp = None
+        if inc is not None:
+            inc = np.ascontiguousarray(inc, np.int32)
+            incp = _p(inc)
+        tsp = None
+        if tile_sel is not None:
+            tile_sel = np.ascontiguousarray(tile_sel, np.uint32)
+            tsp = _p(tile_sel, _u32p)
+        _check(lib.wsbb_multi_run(self.h, kind, incp, margin, tsp, reps, _p(out), _p(st, _i64p),
+                                  ctypes.byref(ms)), "multi_run")
+        return out, dict(nodes=int(st[0]), fallbacks=int(st[1]), pairs=int(st[2])), ms.value

@@ -10,7 +10,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwsbb_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["instance.cpp", "bound.hip", "search.hip", "selector.hip", "capi.hip"]
+SOURCES = ["instance.cpp", "bound.hip", "search.hip", "selector.hip", "multi.hip", "capi.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
     "-Wno-unused-result",

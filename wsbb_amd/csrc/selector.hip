@@ -56,8 +56,13 @@ constexpr int kMaxFsteps = 15;  // ceil(5m / 8) for m <= 24
 
 struct MlpArgs {
   int N, P, nsw, topk, fsteps;
+  int fstride;              // feature steps between 32-node blocks of xfrag
+  int tstride;              // words between tiles of tile_sel (>= nsw)
   const int* Ndev;          // device node count, or null
-  const uint2* xfrag;       // [2 ntiles][fsteps][64] feature B fragments
+  const int* tile_list;     // tiles to score (nlist of them), or null: all
+  int nlist;
+  const int* tile_cnt;      // [ntiles] nodes per tile (with tile_list), or null
+  const uint2* xfrag;       // [2 ntiles][fstride][64] feature B fragments
   const uint2* frag1;       // [4][fsteps][64] A fragments of W1^T
   const uint2* frag2;       // [PB][16][64]    A fragments of W2^T
   const float* b1;          // [128]
@@ -104,19 +109,22 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
   const int col = lane & 31, half = lane >> 5;
-  for (int tile0 = blockIdx.x * 4; tile0 < ntiles; tile0 += gridDim.x * 4) {
-    const int tile = __builtin_amdgcn_readfirstlane(tile0 + wave);
-    if (tile >= ntiles) continue;  // no block barrier below
+  const int nwork = A.tile_list ? A.nlist : ntiles;
+  for (int w0 = blockIdx.x * 4; w0 < nwork; w0 += gridDim.x * 4) {
+    const int wi = __builtin_amdgcn_readfirstlane(w0 + wave);
+    if (wi >= nwork) continue;  // no block barrier below
+    const int tile = A.tile_list ? __builtin_amdgcn_readfirstlane(A.tile_list[wi]) : wi;
+    const int tcnt = A.tile_cnt ? A.tile_cnt[tile] : 64;
     float agg[PB];  // tile sums of the probabilities of pairs 32 bp + col
 #pragma unroll
     for (int bp = 0; bp < PB; ++bp) agg[bp] = 0.f;
     for (int nb = 0; nb < 2; ++nb) {
       const int blk = tile * 2 + nb;
       const int node = blk * 32 + col;
-      const bool valid = node < N;
+      const bool valid = A.tile_list ? (nb * 32 + col < tcnt) : (node < N);
       // layer 1: H^T[128 x 32] = W1^T[128 x F] . X^T[F x 32]; the next
       // feature fragment is loaded while this step multiplies
-      const uint2* xp = A.xfrag + static_cast<size_t>(blk) * A.fsteps * 64 + lane;
+      const uint2* xp = A.xfrag + static_cast<size_t>(blk) * A.fstride * 64 + lane;
       // the accumulators start from the biases: row 32bi + 8q + 4half + r of
       // the lane sits in acc1[bi][4q + r]
       f32x16 acc1[4];
@@ -254,8 +262,10 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
     for (int bp = 0; bp < PB; ++bp) {
       const unsigned long long bal = __ballot(mine[bp] != 0);
       if (lane == 0 && bp < A.nsw)
-        A.tile_sel[static_cast<size_t>(tile) * A.nsw + bp] = static_cast<uint32_t>(bal);
+        A.tile_sel[static_cast<size_t>(tile) * A.tstride + bp] = static_cast<uint32_t>(bal);
     }
+    if (lane == 0)
+      for (int w = A.nsw; w < A.tstride; ++w) A.tile_sel[static_cast<size_t>(tile) * A.tstride + w] = 0u;
   }
 }
 
@@ -533,7 +543,7 @@ static hipError_t refresh_frags(wsbb_ctx* ctx) {
 }
 
 hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int* Ndev,
-                             uint32_t* tile_sel, float* scores, int ld) {
+                             uint32_t* tile_sel, float* scores, int ld, const MlpTiles* tl) {
   SelectorState* S = ctx->selector;
   hipError_t e = refresh_frags(ctx);
   if (e != hipSuccess) return e;
@@ -544,6 +554,11 @@ hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int
   A.nsw = S->nsw;
   A.topk = S->topk;
   A.fsteps = S->Fsteps;
+  A.fstride = tl ? tl->fstride : S->Fsteps;
+  A.tstride = tl ? tl->tstride : S->nsw;
+  A.tile_list = tl ? tl->list : nullptr;
+  A.nlist = tl ? tl->nlist : 0;
+  A.tile_cnt = tl ? tl->cnt : nullptr;
   A.xfrag = xfrag;
   A.frag1 = S->frag1;
   A.frag2 = S->frag2;
@@ -552,7 +567,8 @@ hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int
   A.tile_sel = tile_sel;
   A.scores = scores;
   A.ld = ld;
-  const int ntiles = (N + 63) / 64;
+  const int ntiles = tl ? tl->nlist : (N + 63) / 64;
+  if (ntiles <= 0) return hipSuccess;
   // one persistent block per CU: the weights are staged once per block
   const int grid = std::max(1, std::min((ntiles + 3) / 4, ctx->num_cus));
   const size_t shmem = sizeof(uint2) * (4 * S->Fsteps * 64 + S->PB * 16 * 64) +
@@ -890,7 +906,8 @@ int wsbb_selector_infer(wsbb_ctx* ctx, const float* feat, int N, float* scores, 
   hipLaunchKernelGGL(k_feat_pack, dim3(std::min<long long>((xb / 8 + 255) / 256, 4096)), dim3(256), 0,
                      ctx->stream, dfeat, N, N, S->F, S->Fsteps, dx);
   SEL_HIP(hipGetLastError(), "pack launch");
-  SEL_HIP(launch_mlp_select(ctx, dx, N, nullptr, dts, scores ? dsc : nullptr, N), "mlp launch");
+  SEL_HIP(launch_mlp_select(ctx, dx, N, nullptr, dts, scores ? dsc : nullptr, N, nullptr),
+          "mlp launch");
   if (scores) SEL_HIP(hipMemcpyAsync(scores, dsc, sb, hipMemcpyDeviceToHost, ctx->stream), "D2H scores");
   if (tile_sel) SEL_HIP(hipMemcpyAsync(tile_sel, dts, tb, hipMemcpyDeviceToHost, ctx->stream), "D2H sel");
   SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
