@@ -10,7 +10,9 @@ time-to-optimum.
 
 Also reported ("batched"): the 50x20 batched bounding throughput, 2^20
 synthetic nodes (prefix/suffix lengths uniform, random permutations) per
-launch, inputs resident in HBM.
+launch, inputs resident in HBM; ("learned_20x20"): plain against learned
+pair selection on ta021 solved to optimum; ("mixed_batch"): 64 instances of
+uneven size bounded together, plain and learned, over 1e8 nodes.
 
 The CPU baseline is the oracle (the C restatement of the reference CPU path)
 on this host: the same ta011 solve with the reference's DFS explorer, one
@@ -105,6 +107,123 @@ def bench_batched(wsbb_amd, reps=20, autotune=True):
                 p=p)
 
 
+def bench_learned(wsbb_amd):
+    """20x20 (ta021): time-to-proven-optimum with plain LB2 against learned
+    pair selection (top-16 pairs per 64-child tile, no fallback), the MLP
+    trained on the GPU on (features, LB2-argmax pair) records logged by a
+    plain solve of the same instance.  Hit rate: share of 2^16 random nodes
+    whose tile-subset bound equals full LB2."""
+    p = wsbb_amd.taillard(21)
+    pool, nb, topk = 65536, 4, 16
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=1)
+        eng.selector_log(stride=64, capacity=1 << 19)
+        eng.solve(pool_size=pool, max_batches=nb)
+        feat, label = eng.selector_records()
+        eng.selector_log(stride=0, capacity=0)
+        eng.selector_set_records(feat, label)
+        t = time.perf_counter()
+        losses = eng.selector_train(epochs=4, batch=1024, lr=2e-3, seed=2)
+        train_s = time.perf_counter() - t
+        eng.solve(pool_size=pool, max_batches=nb, budget=20_000)  # warm both paths
+        t = time.perf_counter()
+        plain = eng.solve(pool_size=pool, max_batches=nb)
+        plain_s = time.perf_counter() - t
+        eng.selector_config(topk=topk, margin=-1)
+        eng.solve(kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb, budget=20_000)
+        t = time.perf_counter()
+        learned = eng.solve(kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
+        learned_s = time.perf_counter() - t
+        permT, l1, l2 = wsbb_amd.random_nodes(20, 1 << 16, 0xC0FFEE)
+        full = eng.bound(permT, l1, l2)
+        fe = eng.bound(permT, l1, l2, kind=wsbb_amd.LB1, feat=True)["feat"]
+        _, tsel = eng.selector_infer(fe)
+        sub, _ = eng.bound_tiles(permT, l1, l2, tsel, margin=-1)
+    ps, ls = plain["stats"], learned["stats"]
+    return {
+        "workload": "ta021 20x20 solved to proven optimum, 65536-child pools, 4 sub-batches",
+        "selector": f"MLP 100-128-190, BF16 weights / FP32 accumulation, top-{topk} pairs per "
+                    "64-child tile, trained on records logged by a plain solve of ta021 "
+                    f"({feat.shape[1]} records, 4 epochs, {train_s:.2f} s on the GPU)",
+        "train_loss": [float(x) for x in losses],
+        "plain": {"optimum": plain["best"], "proven": plain["proven"], "seconds": plain_s,
+                  "bounded": ps["bounded"], "pairs_evaluated": ps["pairs_evaluated"]},
+        "learned": {"optimum": learned["best"], "proven": learned["proven"], "seconds": learned_s,
+                    "bounded": ls["bounded"], "pairs_evaluated": ls["pairs_evaluated"]},
+        "speedup_time_to_optimum": plain_s / learned_s,
+        "node_inflation": ls["bounded"] / ps["bounded"],
+        "hit_rate_random_nodes": float((sub == full).mean()),
+    }
+
+
+def bench_mixed(wsbb_amd, budget=100_000_000):
+    """64 instances, n in {20, 50, 100} x m in {5, 10, 20}, job-correlated
+    times, 16384 random nodes each, bounded together in one launch per pass
+    until `budget` nodes are bounded: plain LB2 and learned pair selection
+    (one MLP per machine count, trained on records of a plain solve of the
+    first instance of that size, top-k = max(4, P/8) pairs per tile) with
+    full LB2 for nodes within 30 of their instance's incumbent."""
+    import itertools
+    shapes = list(itertools.product((20, 50, 100), (5, 10, 20)))
+    ps = [wsbb_amd.generate_correlated(*shapes[i % len(shapes)], 0xA11CE + i) for i in range(64)]
+    per = 16384
+    maxn = max(p.shape[1] for p in ps)
+    N = per * len(ps)
+    permT = np.zeros((maxn, N), np.uint8)
+    l1 = np.zeros(N, np.int32)
+    l2 = np.zeros(N, np.int32)
+    offsets = np.arange(len(ps) + 1, dtype=np.int32) * per
+    for i, p in enumerate(ps):
+        pt, a, b = wsbb_amd.random_nodes(p.shape[1], per, 0x5EED + 977 * i)
+        permT[:p.shape[1], i * per:(i + 1) * per] = pt
+        l1[i * per:(i + 1) * per] = a
+        l2[i * per:(i + 1) * per] = b
+    with wsbb_amd.Multi(ps) as mb:
+        inc = np.zeros(len(ps), np.int32)
+        seen = {}
+        t = time.perf_counter()
+        for i, p in enumerate(ps):
+            eng = mb.engines[i]
+            m = p.shape[0]
+            logging = m not in seen
+            if logging:
+                eng.selector_init(seed=3)
+                eng.selector_log(stride=16, capacity=1 << 17)
+            eng.set_pool_capacity(1 << 20)  # at most 5000 parents x 100 children
+            # incumbent: a dive with one parent per round reaches leaves after
+            # about n rounds
+            inc[i] = eng.solve(pool_size=256, max_batches=1, budget=400)["best"]
+            if logging:
+                eng.solve(pool_size=16384, max_batches=2, budget=5_000)
+                f, lab = eng.selector_records()
+                eng.selector_log(stride=0, capacity=0)
+                eng.selector_set_records(f, lab)
+                eng.selector_train(epochs=3, batch=1024, lr=2e-3, seed=2)
+                P = m * (m - 1) // 2
+                eng.selector_config(topk=max(4, P // 8), margin=30)
+                seen[m] = i
+        prep_s = time.perf_counter() - t
+        mb.load(permT, l1, l2, offsets)
+        reps = -(-budget // N)
+        full, st_p, ms_p = mb.run(kind=wsbb_amd.LB2, reps=reps)
+        sub, st_l, ms_l = mb.run(kind=wsbb_amd.LEARNED, inc=inc, margin=30, reps=reps)
+    return {
+        "workload": f"64 instances (n x m over {{20,50,100}} x {{5,10,20}}, p_ij = a_j + U[0,20], "
+                    f"a_j ~ U[1,80]), {per} random nodes each, {N} per launch, {reps} passes "
+                    f"= {reps * N} nodes",
+        "plain": {"nodes_per_s": N / (ms_p / 1e3), "ms_per_pass": ms_p,
+                  "pairs_per_node": st_p["pairs"] / max(st_p["nodes"], 1)},
+        "learned": {"nodes_per_s": N / (ms_l / 1e3), "ms_per_pass": ms_l,
+                    "pairs_per_node": st_l["pairs"] / max(st_l["nodes"], 1),
+                    "fallback_share": st_l["fallbacks"] / max(st_l["nodes"], 1),
+                    "valid_bound": bool((sub <= full).all()),
+                    "hit_rate": float((sub == full).mean())},
+        "incumbents": "best leaf of a 400-parent depth-first dive per instance",
+        "incumbent_found": int((inc < 2**31 - 1).sum()),
+        "setup_s": prep_s,
+    }
+
+
 def cpu_baselines(bt):
     import oracle
     p = oracle.taillard(11)[3]
@@ -118,7 +237,13 @@ def cpu_baselines(bt):
     t0 = time.perf_counter()
     ref = ob.bound_batch(bt["permT"][:, :S], bt["l1"][:S], bt["l2"][:S], threads=1)
     dtb = time.perf_counter() - t0
-    return solve, dict(value=S / dtb, seconds=dtb, sample=S, ref=ref)
+    cores = min(os.cpu_count() or 1, 16)
+    S2 = 1 << 17
+    t0 = time.perf_counter()
+    ob.bound_batch(bt["permT"][:, :S2], bt["l1"][:S2], bt["l2"][:S2], threads=cores)
+    dtc = time.perf_counter() - t0
+    return solve, dict(value=S / dtb, seconds=dtb, sample=S, ref=ref,
+                       all_cores=dict(value=S2 / dtc, cores=cores, sample=S2))
 
 
 def main():
@@ -130,6 +255,8 @@ def main():
                     help="solve = ta011 time-to-optimum only; batched = 50x20 launches only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the 20x20 learned-selector and mixed-batch measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build targets a single MI355X "
@@ -221,6 +348,14 @@ def main():
             line["batched"]["cpu_baseline"] = {
                 "value": cpu_b["value"], "unit": "nodes/s", "cores": 1, "kind": "port",
                 "sample": f"first {cpu_b['sample']} nodes of the same batch, oracle LB2, 1 thread"}
+            ac = cpu_b["all_cores"]
+            line["batched"]["cpu_baseline_all_cores"] = {
+                "value": ac["value"], "unit": "nodes/s", "cores": ac["cores"], "kind": "port",
+                "sample": f"first {ac['sample']} nodes of the same batch, oracle LB2, "
+                          f"{ac['cores']} threads"}
+    if args.workload == "all" and not args.no_learned:
+        line["learned_20x20"] = bench_learned(wsbb_amd)
+        line["mixed_batch"] = bench_mixed(wsbb_amd)
     print(json.dumps(line), flush=True)
 
 
