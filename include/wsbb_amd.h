@@ -143,6 +143,11 @@ int wsbb_timer_stop(wsbb_ctx *ctx, float *ms);
 int wsbb_set_launch(wsbb_ctx *ctx, int groups, int grid_blocks);
 int wsbb_autotune(wsbb_ctx *ctx, int N, int reps, int32_t *shapes,
                   float *ms, int *count);
+/* Plain LB2 (no optional outputs, best = -1) may run two nodes per lane in
+ * int16 halves when every bound value of the instance stays below 2^14:
+ * mode 1 (default) for batches of at least 2 x 128 x (4 x CUs) nodes, 2 for
+ * every batch, 0 never.  Both forms give identical values. */
+int wsbb_set_int16(wsbb_ctx *ctx, int mode);
 
 /* ---- device-resident branch-and-bound -------------------------------------------
  * Replaces explorer::_explore_cpu_gpu (src/explorer.cu:154-305) with

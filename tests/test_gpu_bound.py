@@ -42,6 +42,8 @@ def cases():
            ("corr_50x10", oracle.generate_correlated(50, 10, 5)),
            ("ties_30x6", rng.integers(1, 4, size=(6, 30)).astype(np.int32)),
            ("max_128x24", rng.integers(1, 100, size=(24, 128)).astype(np.int32)),
+           # bound values beyond 2^14: the int16 two-node form does not apply
+           ("big_p_20x10", rng.integers(1, 3000, size=(10, 20)).astype(np.int32)),
            ("tiny_3x2", rng.integers(1, 10, size=(2, 3)).astype(np.int32))]
     return out
 
@@ -59,13 +61,17 @@ def test_lb2_lb1_parity(name, p):
     o = oracle.Oracle(p)
     N = 3000 if n <= 50 else 1000
     permT, l1, l2 = batch_for(n, N, 0x5EED)
+    ref = o.bound_batch(permT, l1, l2, kind=oracle.LB2, threads=8)
     with wsbb_amd.Engine(p) as eng:
-        for groups in (1, 2, 4):
-            eng.set_launch(groups=groups)
-            got = eng.bound(permT, l1, l2, kind=wsbb_amd.LB2)
-            ref = o.bound_batch(permT, l1, l2, kind=oracle.LB2, threads=8)
-            bad = np.nonzero(got != ref)[0]
-            assert bad.size == 0, (groups, bad[:5], got[bad[:5]], ref[bad[:5]])
+        # both the two-node int16 form (when the instance fits) and the int32 form
+        for int16 in (2, 0):
+            eng.set_int16(int16)
+            for groups in (1, 2, 4):
+                eng.set_launch(groups=groups)
+                got = eng.bound(permT, l1, l2, kind=wsbb_amd.LB2)
+                bad = np.nonzero(got != ref)[0]
+                assert bad.size == 0, (int16, groups, bad[:5], got[bad[:5]], ref[bad[:5]])
+        eng.set_int16(1)
         eng.set_launch(groups=0)
         got1 = eng.bound(permT, l1, l2, kind=wsbb_amd.LB1)
         ref1 = o.bound_batch(permT, l1, l2, kind=oracle.LB1, threads=8)

@@ -71,6 +71,7 @@ def _load():
                                 ctypes.c_int),
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_search_options": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_set_int16": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_multi_create": ([ctypes.c_int, ctypes.POINTER(_vp), _i32p, _i32p, ctypes.c_int,
                                ctypes.POINTER(_vp)], ctypes.c_int),
         "wsbb_multi_destroy": ([_vp], ctypes.c_int),
@@ -295,6 +296,11 @@ class Engine:
 
     def set_pool_capacity(self, nodes):
         _check(lib.wsbb_set_pool_capacity(self.h, nodes), "set_pool_capacity")
+
+    def set_int16(self, mode):
+        """Two nodes per lane in int16 halves for plain LB2: 0 never, 1 for
+        large batches (default), 2 whenever the instance fits."""
+        _check(lib.wsbb_set_int16(self.h, int(mode)), "set_int16")
 
     def search_options(self, use_graph=-1, rounds_per_sync=-1):
         """hipGraph replay of the search rounds (1/0) and rounds per host

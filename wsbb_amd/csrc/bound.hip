@@ -362,6 +362,227 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   }
 }
 
+// ---- two nodes per lane in int16 halves ------------------------------------
+// Used for plain LB2 (no optional outputs, no `best` exit) when every bound
+// value of the instance stays below 2^14: lane l of a 128-node tile bounds
+// nodes 128 t + l (low halves) and 128 t + 64 + l (high halves).  Heads and
+// tails are built per node as in bound_kernel and packed; a Johnson step is
+// then two bit extracts (plus word selects for n > 32), one pack, three
+// v_pk_mad_u16 and a v_pk_max_i16 for both nodes, with no scalar work.  The
+// arithmetic is exact: every value, including the clock biased by -2^14,
+// fits a signed 16-bit half.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 as_s16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
+// four steps for two nodes: the job's bit of each node's 32-bit mask word
+// (word chosen by the step's select masks, as in the one-node form), packed
+// into the two halves, then three packed multiply-adds and a packed max
+template <int NW>
+__device__ __forceinline__ void step4_pk(const int4 (&rec)[4], const uint2 (&sel)[4],
+                                         const uint32_t (&uA)[NW], const uint32_t (&uB)[NW],
+                                         s16x2& T0, s16x2& t1) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int4 qq = rec[u];
+    uint32_t wA, wB;
+    if constexpr (NW == 1) {
+      wA = uA[0];
+      wB = uB[0];
+    } else if constexpr (NW == 2) {
+      wA = bitsel(sel[u].x, uA[1], uA[0]);
+      wB = bitsel(sel[u].x, uB[1], uB[0]);
+    } else {
+      wA = bitsel(sel[u].y, bitsel(sel[u].x, uA[3], uA[2]), bitsel(sel[u].x, uA[1], uA[0]));
+      wB = bitsel(sel[u].y, bitsel(sel[u].x, uB[3], uB[2]), bitsel(sel[u].x, uB[1], uB[0]));
+    }
+    const uint32_t j = static_cast<uint32_t>(qq.x);  // bit offset in its low 5 bits
+    const uint32_t fA = __builtin_amdgcn_ubfe(wA, j, 1u);
+    const uint32_t fB = __builtin_amdgcn_ubfe(wB, j, 1u);
+    const s16x2 f = as_s16((fB << 16) | fA);
+    T0 = f * as_s16(qq.y) + T0;
+    const s16x2 x = f * as_s16(qq.z) + t1;
+    const s16x2 y = f * as_s16(qq.w) + T0;
+    t1 = __builtin_elementwise_max(x, y);
+  }
+}
+
+template <int MM, int NW, class Src>
+__global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, BoundArgs A) {
+  extern __shared__ int32_t lds[];
+  const int n = T.n, m = T.m, P = T.P;
+  const int mn4 = (MM * n + 3) & ~3;
+  int32_t* sp = lds;                                           // [MM][n]
+  uint32_t* part = reinterpret_cast<uint32_t*>(lds + mn4);     // [blockDim]
+  uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);  // [n][128 tpb]
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
+  __syncthreads();
+  const int G = A.groups;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tpb = (blockDim.x >> 6) / G;
+  const int tib = wave / G;
+  const int grp = __builtin_amdgcn_readfirstlane(wave - tib * G);
+  const int s_lo = (grp * P) / G, s_hi = ((grp + 1) * P) / G;
+  const int N = (A.Ndev != nullptr) ? *A.Ndev : A.N;
+  const int ntiles = (N + 127) >> 7;
+  const int nbt = (ntiles + tpb - 1) / tpb;
+  const int bst = 128 * tpb;
+  uint32_t valid_bits[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const int hi = n - 32 * w;
+    valid_bits[w] = hi >= 32 ? 0xffffffffu : (hi > 0 ? ((1u << hi) - 1u) : 0u);
+  }
+  for (int bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
+    const int tile = __builtin_amdgcn_readfirstlane(bt * tpb + tib);
+    int node[2], l1[2], l2[2], sw0[2], sw1[2];
+    bool valid[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      node[h] = tile * 128 + h * 64 + lane;
+      int stride = 0;
+      const uint8_t* col = nullptr;
+      l1[h] = -1;
+      l2[h] = n;
+      sw0[h] = sw1[h] = -1;
+      valid[h] = node[h] < N && src.load(node[h], l1[h], l2[h], col, stride, sw0[h], sw1[h]);
+      if (!valid[h]) {
+        l1[h] = -1;
+        l2[h] = n;
+      }
+      if (grp == 0 && valid[h]) {
+        for (int i0 = 0; i0 < n; i0 += 16) {
+          uint32_t v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (i0 + u < n) pl[(i0 + u) * bst + tib * 128 + h * 64 + lane] = static_cast<uint8_t>(v[u]);
+        }
+      }
+    }
+    if (G > 1) __syncthreads();
+    uint32_t rp[MM], qp[MM], uA[NW], uB[NW];
+    uint32_t d2m = 0;
+    int leafv[2];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) rp[k] = qp[k] = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint8_t* mine = pl + tib * 128 + h * 64 + lane;
+      const int d1 = l1[h] + 1, d2 = n - l2[h];
+      int r[MM], q[MM];
+      uint32_t sm[NW];
+#pragma unroll
+      for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sm[w] = 0;
+      for (int i = 0; i < d1; ++i) {
+        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        set_job_bit<NW>(sm, j);
+        r[0] += sp[j];
+#pragma unroll
+        for (int k = 1; k < MM; ++k) r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
+      }
+      for (int i = n - 1; i >= l2[h]; --i) {
+        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        set_job_bit<NW>(sm, j);
+#pragma unroll
+        for (int k = MM - 1; k >= 0; --k) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
+      }
+      int lv = 0;
+#pragma unroll
+      for (int k = 0; k < MM; ++k) lv = max(lv, r[k] + q[k]);
+      leafv[h] = lv;
+#pragma unroll
+      for (int k = 0; k < MM; ++k) {
+        const int rk = d1 ? r[k] : ld_const(T.R0, k);  // zero past machine m
+        rp[k] |= static_cast<uint32_t>(rk) << (16 * h);
+        qp[k] |= static_cast<uint32_t>(q[k]) << (16 * h);
+      }
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t um = ~sm[w] & valid_bits[w];
+        if (h == 0) uA[w] = um;
+        else uB[w] = um;
+      }
+      d2m |= (d2 ? 0xffffu : 0u) << (16 * h);
+    }
+    s16x2 lb = as_s16(0u);
+    const s16x2 bias = as_s16(static_cast<uint32_t>(kBias16) * 0x10001u);
+    for (int s = s_lo; s < s_hi; ++s) {
+      const int2 ab = ld_const2(T.pairs, s);
+      s16x2 T0 = as_s16(rp[ab.x]) - bias;  // machine-a clocks, biased below zero
+      s16x2 t1 = as_s16(rp[ab.y]);
+      const size_t base = static_cast<size_t>(s) * T.n4;
+      for (int k = 0; k < T.n4; k += 4) {
+        int4 rec[4];
+        uint2 sel[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          rec[u] = ld_const4(T.steps16, base + k + u);
+          if constexpr (NW > 1) sel[u] = ld_const2u(T.wsel, base + k + u);
+        }
+        step4_pk<NW>(rec, sel, uA, uB, T0, t1);
+      }
+      const uint32_t q0b = static_cast<uint32_t>(ld_const(T.Q0, ab.y)) * 0x10001u;
+      const s16x2 c1 = __builtin_elementwise_max(t1 + as_s16(qp[ab.y]), T0 + bias + as_s16(qp[ab.x]));
+      const s16x2 c0 = t1 + as_s16(q0b);
+      const uint32_t v = (as_u32(c1) & d2m) | (as_u32(c0) & ~d2m);
+      lb = __builtin_elementwise_max(lb, as_s16(v));
+    }
+    if (G > 1) {
+      part[threadIdx.x] = as_u32(lb);
+      __syncthreads();
+      if (grp == 0)
+        for (int g = 1; g < G; ++g) lb = __builtin_elementwise_max(lb, as_s16(part[(tib * G + g) * 64 + lane]));
+      __syncthreads();
+    }
+    if (grp == 0) {
+      const uint32_t lbu = as_u32(lb);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (valid[h]) {
+          const int res = (l2[h] - l1[h] == 1) ? leafv[h] : static_cast<int>((lbu >> (16 * h)) & 0xffffu);
+          A.out[src.slot(node[h])] = res;
+        }
+    }
+  }
+}
+
+template <int MM, int NW, class Src>
+hipError_t launch_pk(const DevTables& T, const Src& src, const BoundArgs& A, int grid_cap,
+                     hipStream_t stream) {
+  const int block = 256;
+  const int tpb = (block / 64) / A.groups;
+  const int ntiles = (A.N + 127) / 128;
+  int grid = (ntiles + tpb - 1) / tpb;
+  if (grid > grid_cap) grid = grid_cap;
+  if (grid < 1) grid = 1;
+  const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + block) +
+                       static_cast<size_t>(T.n) * 128 * tpb;
+  hipLaunchKernelGGL((bound_kernel_pk<MM, NW, Src>), dim3(grid), dim3(block), shmem, stream, T, src, A);
+  return hipGetLastError();
+}
+
+template <class Src>
+hipError_t launch_pk_src(const DevTables& T, const Src& src, const BoundArgs& A, int grid_cap,
+                         hipStream_t stream) {
+  const int nw = T.n <= 32 ? 1 : (T.n <= 64 ? 2 : 4);
+  const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
+#define WSBB_PK(MMv, NWv) \
+  if (mm == MMv && nw == NWv) return launch_pk<MMv, NWv, Src>(T, src, A, grid_cap, stream);
+  WSBB_PK(8, 1) WSBB_PK(8, 2) WSBB_PK(8, 4)
+  WSBB_PK(12, 1) WSBB_PK(12, 2) WSBB_PK(12, 4)
+  WSBB_PK(16, 1) WSBB_PK(16, 2) WSBB_PK(16, 4)
+  WSBB_PK(20, 1) WSBB_PK(20, 2) WSBB_PK(20, 4)
+  WSBB_PK(24, 1) WSBB_PK(24, 2) WSBB_PK(24, 4)
+#undef WSBB_PK
+  return hipErrorInvalidValue;
+}
+
 // Selector features of every node as BF16 B-operand fragments of
 // v_mfma_f32_32x32x8_bf16.  5m features per node: per machine the head (R0
 // without prefix), the tail (Q0 without suffix), the remaining work of the
@@ -518,6 +739,8 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
                                                                    : 1;
   // 8 or 16 waves per tile (1024-thread blocks) for small batches; the lean
   // variants for up to 16 machines fit four such waves per SIMD
+  if (mode == 0 && A.best == -1 && T.steps16 != nullptr && A.groups <= 4 && A.pk != 0)
+    return launch_pk_src(T, src, A, grid_cap, stream);
   bool wide = A.groups > 4;
   if (wide && (mode != 0 || mm > 16 || std::is_same<Src, ListSource>::value)) {
     // the general and wide-machine variants stop at four waves per tile

@@ -181,11 +181,24 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
       steps[at] = int4{j, pa, pb, kBias + I.lag[s * n + j] + pb};
       wsel[at] = uint2{((j >> 5) & 1) ? 0xffffffffu : 0u, ((j >> 6) & 1) ? 0xffffffffu : 0u};
     }
+  // two nodes per lane in int16 halves when every bound value (<= the path
+  // bound) and every step constant stays below 2^14
+  const bool two16 = I.path_bound < kBias16 - 1;
+  std::vector<int4> steps16(two16 ? nsteps : 0, int4{0, 0, 0, 0});
+  if (two16)
+    for (int s = 0; s < P; ++s)
+      for (int k = 0; k < n; ++k) {
+        const size_t at = static_cast<size_t>(s) * n4 + k;
+        const int4 w = steps[at];
+        const int K = kBias16 + (w.w - kBias);  // 2^14 + lag + p_b
+        steps16[at] = int4{w.x * 0x10001, w.y * 0x10001, w.z * 0x10001, K * 0x10001};
+      }
   std::vector<int2> pairs(P);
   for (int s = 0; s < P; ++s) pairs[s] = int2{I.pair_a[s], I.pair_b[s]};
   const size_t o_steps = 0;
   const size_t o_wsel = o_steps + steps.size() * sizeof(int4);
-  const size_t o_pairs = o_wsel + wsel.size() * sizeof(uint2);
+  const size_t o_s16 = o_wsel + wsel.size() * sizeof(uint2);
+  const size_t o_pairs = o_s16 + steps16.size() * sizeof(int4);
   const size_t o_p = (o_pairs + pairs.size() * sizeof(int2) + 255) & ~size_t(255);
   const size_t o_R0 = o_p + ((I.p.size() * 4 + 255) & ~size_t(255));
   const size_t o_Q0 = o_R0 + 256;
@@ -195,6 +208,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   std::vector<char> blob(total, 0);
   std::memcpy(blob.data() + o_steps, steps.data(), steps.size() * sizeof(int4));
   std::memcpy(blob.data() + o_wsel, wsel.data(), wsel.size() * sizeof(uint2));
+  if (two16) std::memcpy(blob.data() + o_s16, steps16.data(), steps16.size() * sizeof(int4));
   std::memcpy(blob.data() + o_pairs, pairs.data(), pairs.size() * sizeof(int2));
   std::memcpy(blob.data() + o_p, I.p.data(), I.p.size() * 4);
   std::memcpy(blob.data() + o_R0, I.R0.data(), m * 4);
@@ -214,6 +228,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   T.R0 = ctx->tables.as<const int32_t>(o_R0);
   T.Q0 = ctx->tables.as<const int32_t>(o_Q0);
   T.ptot = ctx->tables.as<const int32_t>(o_tot);
+  T.steps16 = two16 ? ctx->tables.as<const int4>(o_s16) : nullptr;
   *out = ctx;
   return WSBB_OK;
 }
@@ -296,6 +311,7 @@ int wsbb_bound_batch_ex(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
   A.kind = kind;
   A.best = best;
   A.groups = ctx->groups_for(N);
+  A.pk = ctx->int16_for(N);
   A.out = ctx->batch.as<int32_t>(o_out);
   A.argmax = argmax ? ctx->batch.as<int32_t>(o_am) : nullptr;
   A.pairv = pairv ? ctx->batch.as<int32_t>(o_pv) : nullptr;
@@ -378,6 +394,7 @@ int wsbb_bound_device(wsbb_ctx* ctx, const uint8_t* d_permT, const int32_t* d_l1
   A.kind = kind;
   A.best = -1;
   A.groups = ctx->groups_for(N);
+  A.pk = ctx->int16_for(N);
   A.out = d_out;
   WSBB_HIP(launch_bound(ctx->T, src, A, ctx->grid_cap(), ctx->stream), "bound kernel launch");
   return WSBB_OK;
@@ -449,6 +466,16 @@ int wsbb_set_launch(wsbb_ctx* ctx, int groups, int grid_blocks) {
   return WSBB_OK;
 }
 
+int wsbb_set_int16(wsbb_ctx* ctx, int mode) {
+  WSBB_CHECK_CTX(ctx);
+  if (mode < 0 || mode > 2) {
+    set_error("int16 mode must be 0 (never), 1 (large batches) or 2 (always when it fits)");
+    return WSBB_EINVAL;
+  }
+  ctx->launch.int16 = mode;
+  return WSBB_OK;
+}
+
 int wsbb_autotune(wsbb_ctx* ctx, int N, int reps, int32_t* shapes, float* ms,
                   int* count) {
   WSBB_CHECK_CTX(ctx);
@@ -494,6 +521,7 @@ int wsbb_autotune(wsbb_ctx* ctx, int N, int reps, int32_t* shapes, float* ms,
       A.kind = 0;
       A.best = -1;
       A.groups = g;
+      A.pk = ctx->int16_for(N);
       A.out = static_cast<int32_t*>(dout);
       hipError_t e = launch_bound(ctx->T, src, A, grid, ctx->stream);  // warm-up
       if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);

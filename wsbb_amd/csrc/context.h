@@ -46,6 +46,8 @@ struct SelectorState;  // selector.hip
 struct Launch {
   int groups = 1;
   int grid_cap = 0;  // 0: automatic
+  int int16 = 1;     // two nodes per lane in int16 halves: 0 never, 1 for
+                     // large batches, 2 whenever the instance fits
 };
 
 }  // namespace wsbb
@@ -74,4 +76,10 @@ struct wsbb_ctx {
     return num_cus * 8;
   }
   int groups_for(int N) const;
+  // two nodes per lane pays once a batch gives every SIMD two 128-node tiles;
+  // smaller launches are latency-bound and prefer more wavefronts
+  int int16_for(long long N) const {
+    if (launch.int16 == 2) return 1;
+    return launch.int16 == 1 && N >= 2LL * 128 * 4 * num_cus;
+  }
 };

@@ -41,6 +41,9 @@ struct DevTables {
   const int32_t* R0;     // [m]
   const int32_t* Q0;     // [m]
   const int32_t* ptot;   // [m]
+  const int4* steps16;   // [P][n4] two-node form {job * 0x10001, p_a * 0x10001,
+                         // p_b * 0x10001, (2^14 + lag + p_b) * 0x10001}, or
+                         // null when bound values may reach 2^14
 };
 
 // What a bound launch computes and where it writes.
@@ -65,6 +68,7 @@ struct BoundArgs {
   const int* inc_dev;        // device incumbent for the margin test
   long long* stats;          // with tile_sel: [0] nodes, [1] full-LB2
                              // fallbacks, [2] node-pair evaluations
+  int pk = 1;                // plain LB2 may run two nodes per lane (int16)
 };
 
 // Node source: explicit position-major nodes.

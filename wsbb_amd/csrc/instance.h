@@ -10,6 +10,9 @@ namespace wsbb {
 // Offset that keeps the biased machine-a clock (t0 - kBias) negative and the
 // 24-bit multiply-add operands in range; every bound value must stay below it.
 constexpr int kBias = 1 << 22;
+// The same offset for the two-node int16 form, used when every bound value
+// of the instance stays below it.
+constexpr int kBias16 = 1 << 14;
 constexpr int kMaxJobs = 128;     // 4 x 32-bit job masks per node
 constexpr int kMaxMachines = 24;  // register arrays for heads/tails
 

@@ -892,6 +892,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   }
   B.out = A.c_bound;
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
+  B.pk = ctx->int16_for(A.CCAP);
   hipError_t e;
   if (kind == WSBB_LEARNED) {
     // BF16 features of every child, MLP pair subset per 64-child tile, LB2
@@ -926,6 +927,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     Bf.groups = B.groups > 1 ? B.groups : 4;
     Bf.margin = -1;
     Bf.out = A.c_bound;
+    Bf.pk = ctx->int16_for(A.CCAP);
     if ((e = launch_bound(ctx->T, ls, Bf, ctx->grid_cap(), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_fallback_stats, dim3(1), dim3(1), 0, st, A);
   }
