@@ -57,6 +57,7 @@ constexpr int kMaxFsteps = 15;  // ceil(5m / 8) for m <= 24
 struct MlpArgs {
   int N, P, nsw, topk, fsteps;
   int fstride;              // feature steps between 32-node blocks of xfrag
+  int wstride;              // uint2 words of LDS per wave (features / sums)
   int tstride;              // words between tiles of tile_sel (>= nsw)
   const int* Ndev;          // device node count, or null
   const int* tile_list;     // tiles to score (nlist of them), or null: all
@@ -81,23 +82,26 @@ __device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
   return r;
 }
 
-// Four waves per block, one 64-node tile per wave and iteration; the weight
-// fragments of both layers are staged in LDS once per block.
-template <int PB>
-__global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
+// One persistent block per CU with eight waves (four when the LDS cannot hold
+// eight waves' areas), one 64-node tile per wave and iteration; the weight
+// fragments of both layers are staged in LDS once per block, and each
+// block's feature fragments pass through the wave's LDS area so that every
+// feature load of a block is in flight at once.
+template <int PB, bool SCORES>
+__global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   extern __shared__ uint2 s_w[];
   const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
   // 16 loads in flight per thread before their LDS stores
-  for (int i0 = 0; i0 < n1 + n2; i0 += 16 * 256) {
+  for (int i0 = 0; i0 < n1 + n2; i0 += 16 * blockDim.x) {
     uint2 v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const int i = i0 + u * 256 + threadIdx.x;
+      const int i = i0 + u * blockDim.x + threadIdx.x;
       if (i < n1 + n2) v[u] = i < n1 ? A.frag1[i] : A.frag2[i - n1];
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const int i = i0 + u * 256 + threadIdx.x;
+      const int i = i0 + u * blockDim.x + threadIdx.x;
       if (i < n1 + n2) s_w[i] = v[u];
     }
   }
@@ -105,12 +109,16 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
   const uint2* w1 = s_w;
   const uint2* w2 = s_w + n1;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float* s_p = reinterpret_cast<float*>(s_w + n1 + n2) + wave * 32 * 33;  // [32 pairs][33]
+  const int nwaves = blockDim.x >> 6;
+  // per wave: the feature fragments of one 32-node block [fsteps][64] for
+  // layer 1, then (reused) the probabilities of 32 pairs [32][33]
+  uint2* s_x = s_w + n1 + n2 + static_cast<size_t>(wave) * A.wstride;
+  float* s_p = reinterpret_cast<float*>(s_x);
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
   const int col = lane & 31, half = lane >> 5;
   const int nwork = A.tile_list ? A.nlist : ntiles;
-  for (int w0 = blockIdx.x * 4; w0 < nwork; w0 += gridDim.x * 4) {
+  for (int w0 = blockIdx.x * nwaves; w0 < nwork; w0 += gridDim.x * nwaves) {
     const int wi = __builtin_amdgcn_readfirstlane(w0 + wave);
     if (wi >= nwork) continue;  // no block barrier below
     const int tile = A.tile_list ? __builtin_amdgcn_readfirstlane(A.tile_list[wi]) : wi;
@@ -122,9 +130,21 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
       const int blk = tile * 2 + nb;
       const int node = blk * 32 + col;
       const bool valid = A.tile_list ? (nb * 32 + col < tcnt) : (node < N);
-      // layer 1: H^T[128 x 32] = W1^T[128 x F] . X^T[F x 32]; the next
-      // feature fragment is loaded while this step multiplies
+      // the block's feature fragments go to LDS with every load in flight at
+      // once (one memory latency per block instead of one per step)
       const uint2* xp = A.xfrag + static_cast<size_t>(blk) * A.fstride * 64 + lane;
+      __builtin_amdgcn_sched_barrier(0);
+      {
+        uint2 xv[kMaxFsteps];
+#pragma unroll
+        for (int t = 0; t < kMaxFsteps; ++t)
+          if (t < A.fsteps) xv[t] = xp[t * 64];
+#pragma unroll
+        for (int t = 0; t < kMaxFsteps; ++t)
+          if (t < A.fsteps) s_x[t * 64 + lane] = xv[t];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // layer 1: H^T[128 x 32] = W1^T[128 x F] . X^T[F x 32]
       // the accumulators start from the biases: row 32bi + 8q + 4half + r of
       // the lane sits in acc1[bi][4q + r]
       f32x16 acc1[4];
@@ -132,16 +152,15 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc1[bi][v] = A.b1[32 * bi + 8 * (v >> 2) + 4 * half + (v & 3)];
-      uint2 xn = xp[0];
 #pragma unroll 1
       for (int t = 0; t < A.fsteps; ++t) {
-        const bf16x4 b = as_bf16x4(xn);
-        if (t + 1 < A.fsteps) xn = xp[(t + 1) * 64];
+        const bf16x4 b = as_bf16x4(s_x[t * 64 + lane]);
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
           acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
               as_bf16x4(w1[(bi * A.fsteps + t) * 64 + lane]), b, acc1[bi], 0, 0, 0);
       }
+      __builtin_amdgcn_wave_barrier();  // s_x is reused for s_p below
       // bias, ReLU, BF16: lane holds hidden 32bi + 8q + 4half + r for its node,
       // which is the layer-2 B operand of k-step 4bi + q
       bf16x4 hb[16];
@@ -181,7 +200,7 @@ __global__ __launch_bounds__(256) void k_mlp_select(MlpArgs A) {
         __builtin_amdgcn_sched_barrier(0);
       }
       // logits; this lane's pairs are 32bp + 8(v/4) + 4half + v%4
-      if (A.scores != nullptr) {
+      if constexpr (SCORES) {
         int P = A.P;
         asm volatile("" : "+s"(P));  // keep the bounds tests out of the loop preheader
 #pragma unroll
@@ -569,14 +588,20 @@ hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int
   A.ld = ld;
   const int ntiles = tl ? tl->nlist : (N + 63) / 64;
   if (ntiles <= 0) return hipSuccess;
-  // one persistent block per CU: the weights are staged once per block
-  const int grid = std::max(1, std::min((ntiles + 3) / 4, ctx->num_cus));
-  const size_t shmem = sizeof(uint2) * (4 * S->Fsteps * 64 + S->PB * 16 * 64) +
-                       sizeof(float) * 4 * 32 * 33;
+  // one persistent block per CU: the weights are staged once per block, and
+  // eight waves (two per SIMD) share them when the LDS holds their areas
+  A.wstride = std::max(S->Fsteps * 64, (32 * 33 + 1) / 2);
+  const size_t wbytes = sizeof(uint2) * (4 * S->Fsteps * 64 + S->PB * 16 * 64);
+  const int waves = wbytes + sizeof(uint2) * 8 * A.wstride <= 160 * 1024 ? 8 : 4;
+  const int grid = std::max(1, std::min((ntiles + waves - 1) / waves, ctx->num_cus));
+  const size_t shmem = wbytes + sizeof(uint2) * waves * A.wstride;
   switch (S->PB) {
-#define WSBB_PB(v)                                                                       \
-  case v:                                                                                \
-    hipLaunchKernelGGL(k_mlp_select<v>, dim3(grid), dim3(256), shmem, ctx->stream, A); \
+#define WSBB_PB(v)                                                                          \
+  case v:                                                                                   \
+    if (scores)                                                                             \
+      hipLaunchKernelGGL((k_mlp_select<v, true>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_mlp_select<v, false>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
     break;
     WSBB_PB(1) WSBB_PB(2) WSBB_PB(3) WSBB_PB(4) WSBB_PB(5) WSBB_PB(6) WSBB_PB(7) WSBB_PB(8)
     WSBB_PB(9)
