@@ -371,7 +371,9 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S) {
     const int wbase = pb + (threadIdx.x & ~63);
     const int cntw = min(64, np - wbase);  // wave-uniform
     for (int q = 0; q < cntw; ++q) {
-      const int ql1 = __shfl(l1, q, 64), ql2 = __shfl(l2, q, 64), qoff = __shfl(off, q, 64);
+      // q is wave-uniform: read lane q's values into scalar registers
+      const int ql1 = __builtin_amdgcn_readlane(l1, q), ql2 = __builtin_amdgcn_readlane(l2, q),
+                qoff = __builtin_amdgcn_readlane(off, q);
       const int tag = (wbase + q) << 9;
       const int u = ql2 - ql1 - 1;
       if ((ql1 + 1) + (n - ql2) == n - 2) {  // simple parent: two leaves, BEGIN side
