@@ -107,16 +107,46 @@ def bench_batched(wsbb_amd, reps=20, autotune=True):
                 p=p)
 
 
+def children_of(permT, l1, l2):
+    """Every child of the given nodes in the search's order (per parent the
+    BEGIN children, then the END children; decompose_beginend semantics):
+    (child permT, l1, l2, children per parent)."""
+    n, N = permT.shape
+    cols, a, b, counts = [], [], [], []
+    for k in range(N):
+        u = l2[k] - l1[k] - 1
+        cnt = 0
+        for side in (0, 1):
+            pos = l1[k] + 1 if side == 0 else l2[k] - 1
+            for j in range(l1[k] + 1, l2[k]):
+                c = permT[:, k].copy()
+                c[pos], c[j] = c[j], c[pos]
+                ca, cb = l1[k] + (side == 0), l2[k] - (side == 1)
+                if cb - ca == 2:
+                    ca += 1
+                cols.append(c)
+                a.append(ca)
+                b.append(cb)
+                cnt += 1
+        counts.append(cnt)
+        assert u >= 3
+    return (np.stack(cols, axis=1), np.array(a, np.int32), np.array(b, np.int32),
+            np.array(counts, np.int32))
+
+
 def bench_learned(wsbb_amd):
     """20x20 (ta021): time-to-proven-optimum with plain LB2 against learned
-    pair selection (top-16 pairs per 64-child tile, no fallback), the MLP
-    trained on the GPU on (features, LB2-argmax pair) records logged by a
-    plain solve of the same instance.  Hit rate: share of 2^16 random nodes
-    whose tile-subset bound equals full LB2."""
+    pair selection, no fallback.  The MLP scores the round's parents (the
+    pairs that maximise LB2 over a parent's children); each 64-child tile
+    bounds over the top-16 pairs of its parents' summed probabilities.
+    Trained on the GPU on (parent features, child's LB2-argmax pair) records
+    logged by a plain solve of the same instance.  Hit rate: share of the
+    children of 4096 random parents whose tile-subset bound equals full LB2."""
     p = wsbb_amd.taillard(21)
     pool, nb, topk = 65536, 4, 16
     with wsbb_amd.Engine(p) as eng:
         eng.selector_init(seed=1)
+        eng.selector_config(topk=topk, margin=-1, parents=True)
         eng.selector_log(stride=64, capacity=1 << 19)
         eng.solve(pool_size=pool, max_batches=nb)
         feat, label = eng.selector_records()
@@ -129,22 +159,26 @@ def bench_learned(wsbb_amd):
         t = time.perf_counter()
         plain = eng.solve(pool_size=pool, max_batches=nb)
         plain_s = time.perf_counter() - t
-        eng.selector_config(topk=topk, margin=-1)
         eng.solve(kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb, budget=20_000)
         t = time.perf_counter()
         learned = eng.solve(kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
         learned_s = time.perf_counter() - t
-        permT, l1, l2 = wsbb_amd.random_nodes(20, 1 << 16, 0xC0FFEE)
-        full = eng.bound(permT, l1, l2)
-        fe = eng.bound(permT, l1, l2, kind=wsbb_amd.LB1, feat=True)["feat"]
-        _, tsel = eng.selector_infer(fe)
-        sub, _ = eng.bound_tiles(permT, l1, l2, tsel, margin=-1)
+        permT, l1, l2 = wsbb_amd.random_nodes(20, 4096, 0xC0FFEE)
+        keep = (l2 - l1 - 1) >= 3
+        permT, l1, l2 = permT[:, keep], l1[keep], l2[keep]
+        cpermT, cl1, cl2, counts = children_of(permT, l1, l2)
+        pfeat = eng.bound(permT, l1, l2, kind=wsbb_amd.LB1, feat=True)["feat"]
+        _, tsel = eng.selector_infer_parents(pfeat, counts)
+        full = eng.bound(cpermT, cl1, cl2)
+        sub, _ = eng.bound_tiles(cpermT, cl1, cl2, tsel, margin=-1)
     ps, ls = plain["stats"], learned["stats"]
     return {
         "workload": "ta021 20x20 solved to proven optimum, 65536-child pools, 4 sub-batches",
-        "selector": f"MLP 100-128-190, BF16 weights / FP32 accumulation, top-{topk} pairs per "
-                    "64-child tile, trained on records logged by a plain solve of ta021 "
-                    f"({feat.shape[1]} records, 4 epochs, {train_s:.2f} s on the GPU)",
+        "selector": f"MLP 100-128-190 on each round's parents, BF16 weights / FP32 "
+                    f"accumulation, top-{topk} pairs per 64-child tile from the parents' "
+                    "probabilities weighted by their children in the tile, trained on records "
+                    f"logged by a plain solve of ta021 ({feat.shape[1]} records, 4 epochs, "
+                    f"{train_s:.2f} s on the GPU)",
         "train_loss": [float(x) for x in losses],
         "plain": {"optimum": plain["best"], "proven": plain["proven"], "seconds": plain_s,
                   "bounded": ps["bounded"], "pairs_evaluated": ps["pairs_evaluated"]},
@@ -152,7 +186,8 @@ def bench_learned(wsbb_amd):
                     "bounded": ls["bounded"], "pairs_evaluated": ls["pairs_evaluated"]},
         "speedup_time_to_optimum": plain_s / learned_s,
         "node_inflation": ls["bounded"] / ps["bounded"],
-        "hit_rate_random_nodes": float((sub == full).mean()),
+        "hit_rate_random_children": float((sub == full).mean()),
+        "hit_rate_sample": f"{int(counts.sum())} children of {len(counts)} random parents",
     }
 
 

@@ -228,6 +228,12 @@ int wsbb_selector_params(wsbb_ctx *ctx, int set, float *W1, float *b1,
 /* topk pairs per 64-child tile; children whose subset bound lies in
  * [incumbent - margin, incumbent) get full LB2 (margin -1: none) */
 int wsbb_selector_config(wsbb_ctx *ctx, int topk, int margin);
+/* on = 1: the network works per parent.  Logging then records, for every
+ * stride-th child, its parent's features with the child's label; learned
+ * rounds score the round's parents, and a 64-child tile takes the top-k
+ * pairs of its parents' probabilities weighted by their children in the
+ * tile.  on = 0 (default): features and scores are the children's own. */
+int wsbb_selector_parents(wsbb_ctx *ctx, int on);
 /* log every stride-th child of later plain solves into a device store of
  * `capacity` records (stride 0 stops logging); clears the store */
 int wsbb_selector_log(wsbb_ctx *ctx, int stride, int capacity);
@@ -246,6 +252,13 @@ int wsbb_selector_train(wsbb_ctx *ctx, int epochs, int batch, float lr,
  * top-k masks tile_sel [ceil(N/64)][ceil(P/32)] (may be NULL) */
 int wsbb_selector_infer(wsbb_ctx *ctx, const float *feat, int N,
                         float *scores, uint32_t *tile_sel);
+/* per-parent inference: features feat [5m][Np] of Np parents with counts[p]
+ * consecutive children each; probs [Np][P] (BF16 softmax probabilities as
+ * float, may be NULL) and the masks tile_sel [ceil(sum counts / 64)]
+ * [ceil(P/32)] of the 64-child tiles (may be NULL) */
+int wsbb_selector_infer_parents(wsbb_ctx *ctx, const float *feat, int Np,
+                                const int32_t *counts, float *probs,
+                                uint32_t *tile_sel);
 
 /* ---- mixed batches of several instances -------------------------------------
  * No counterpart in the reference (one instance per process there).  A

@@ -105,6 +105,94 @@ def test_inference_matches_torch_and_tile_topk(tid):
         assert (agg[[s for s in range(eng.P) if s not in chosen]] <= kth + 1e-4).all()
 
 
+@pytest.mark.parametrize("tid", [1, 11, 21, 51])
+def test_parent_inference_and_tile_sums(tid):
+    """Per-parent inference: BF16 softmax probabilities against the torch
+    forward; the mask of every 64-child tile against a numpy top-k of its
+    parents' probabilities weighted by their children in the tile (parents
+    without children, two-leaf parents and tiles spanning many parents
+    included)."""
+    p = oracle.taillard(tid)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=7)
+        params = eng.selector_params()
+        rng = np.random.default_rng(9)
+        params["b2"] = rng.normal(0, 0.5, eng.P).astype(np.float32)
+        eng.selector_params(params)
+        topk = min(16, eng.P)
+        eng.selector_config(topk=topk, margin=0)
+        Np = 700
+        feat = rng.random((5 * eng.m, Np), dtype=np.float32)
+        counts = rng.integers(0, 41, Np).astype(np.int32)
+        counts[rng.random(Np) < 0.2] = 2
+        counts[rng.random(Np) < 0.05] = 0
+        counts[-1] = 0
+        probs, tsel = eng.selector_infer_parents(feat, counts)
+    ref = torch_logits(params, feat)  # [Np][P]
+    e = np.exp(ref - ref.max(axis=1, keepdims=True))
+    pr = e / e.sum(axis=1, keepdims=True)
+    # BF16 rounding (2^-9 relative) on top of the MFMA logits' 2e-3 absolute
+    # error: 1% relative, 1e-6 absolute for underflowing entries
+    assert np.allclose(probs, pr, rtol=1e-2, atol=1e-6)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    Nc = int(off[-1])
+    assert tsel.shape[0] == (Nc + 63) // 64
+    for t in range(tsel.shape[0]):
+        c0, c1 = 64 * t, min(64 * t + 64, Nc)
+        w = np.clip(np.minimum(off[1:], c1) - np.maximum(off[:-1], c0), 0, None)
+        agg = (w[:, None].astype(np.float64) * probs).sum(axis=0)
+        chosen = [s for s in range(eng.P) if (tsel[t, s // 32] >> (s % 32)) & 1]
+        assert len(chosen) == topk
+        kth = np.sort(agg)[::-1][topk - 1]
+        # fp32 sums in the kernel's order: 1e-4 absolute (sums reach 64)
+        assert agg[chosen].min() >= kth - 1e-4
+        assert (agg[[s for s in range(eng.P) if s not in chosen]] <= kth + 1e-4).all()
+
+
+def test_parent_selector_logging_and_solves():
+    """Per-parent mode on 20x10 ta011: records hold the parent's features
+    and the child's label (the first round's records all carry the root's
+    features); learned solves over parent-scored tiles prove the plain
+    optimum with and without the full-LB2 fallback."""
+    p = oracle.taillard(11)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.selector_init(seed=1)
+        eng.selector_config(topk=8, margin=30, parents=True)
+        eng.selector_log(stride=4, capacity=100000)
+        plain = eng.solve(pool_size=4096, max_batches=10)
+        feat, label = eng.selector_records()
+        eng.selector_log(stride=0, capacity=0)
+        eng.selector_set_records(feat, label)
+        assert plain["best"] == 1582 and feat.shape[1] > 1000
+        assert ((label >= -1) & (label < eng.P)).all()
+        # the root round has 2n = 40 children: records 0..9 (stride 4) carry
+        # the root's features, head R0, tail Q0, all the work
+        m, n = p.shape
+        t = oracle.Oracle(p).tables()
+        scale = int(p.max(axis=0).sum() + p.max(axis=1).sum())  # path bound
+        rem = p.sum(axis=1)
+        root = np.concatenate([t["R0"], t["Q0"], rem, t["R0"] + rem, rem + t["Q0"]])
+        for r in range(2 * n // 4):
+            assert np.allclose(feat[:, r] * scale, root, rtol=1e-5, atol=1e-3)
+        # every record: head + work and work + tail blocks are sums of the
+        # first three
+        f = feat.reshape(5, m, -1)
+        assert np.allclose(f[3], f[0] + f[2], atol=1e-5)
+        assert np.allclose(f[4], f[2] + f[1], atol=1e-5)
+        losses = eng.selector_train(epochs=4, batch=512, lr=2e-3, seed=2)
+        assert losses[-1] < losses[0]
+        fb = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+        eng.selector_config(topk=8, margin=-1)
+        nofb = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+    for r in (fb, nofb):
+        assert r["proven"] and r["best"] == plain["best"]
+        assert r["stats"]["subset_bounded"] == r["stats"]["bounded"]
+    assert nofb["stats"]["fallbacks"] == 0
+    print(f"parents: nodes plain {plain['stats']['bounded']} learned+fallback "
+          f"{fb['stats']['bounded']} ({fb['stats']['fallbacks']} fallbacks) learned "
+          f"{nofb['stats']['bounded']}")
+
+
 @pytest.mark.parametrize("tid", [11, 51])
 def test_tile_subset_bounds_match_oracle(tid):
     p = oracle.taillard(tid)[3]
@@ -178,18 +266,20 @@ def test_logging_training_and_learned_solves():
           f"{plain['stats']['bounded']}; fallbacks {st['fallbacks']}")
 
 
-def test_trained_selector_gains_on_20x20():
+@pytest.mark.parametrize("parents", [False, True])
+def test_trained_selector_gains_on_20x20(parents):
     """What the trained network gains over the method without it, on ta021
     (20 jobs x 20 machines, 190 pairs): trained on records of a plain solve
     of the instance, top-16 pairs per tile, the learned search proves the
     same optimum with a small node inflation while evaluating a fraction of
     the pairs; the untrained network (same search, random weights) needs far
-    more nodes."""
+    more nodes.  Per child and per parent (the network scores the round's
+    parents)."""
     p = oracle.taillard(21)[3]
     pool, nb = 65536, 4
     with wsbb_amd.Engine(p) as eng:
         eng.selector_init(seed=1)
-        eng.selector_config(topk=16, margin=-1)
+        eng.selector_config(topk=16, margin=-1, parents=parents)
         untrained = eng.solve(kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb,
                               budget=40_000_000)
         eng.selector_log(stride=64, capacity=1 << 19)

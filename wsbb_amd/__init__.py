@@ -96,6 +96,9 @@ def _load():
         "wsbb_selector_train": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                  _f32p], ctypes.c_int),
         "wsbb_selector_infer": ([_vp, _f32p, ctypes.c_int, _f32p, _u32p], ctypes.c_int),
+        "wsbb_selector_parents": ([_vp, ctypes.c_int], ctypes.c_int),
+        "wsbb_selector_infer_parents": ([_vp, _f32p, ctypes.c_int, _i32p, _f32p, _u32p],
+                                        ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -325,8 +328,13 @@ class Engine:
         arrs = [np.ascontiguousarray(params[k], np.float32) for k in ("W1", "b1", "W2", "b2")]
         _check(lib.wsbb_selector_params(self.h, 1, *[_p(a, _f32p) for a in arrs]), "params")
 
-    def selector_config(self, topk=16, margin=0):
+    def selector_config(self, topk=16, margin=0, parents=None):
+        """parents: True = the network scores parents (records hold the
+        parent's features with each child's label), False = children; None
+        keeps the current setting."""
         _check(lib.wsbb_selector_config(self.h, topk, margin), "selector_config")
+        if parents is not None:
+            _check(lib.wsbb_selector_parents(self.h, int(bool(parents))), "selector_parents")
 
     def selector_log(self, stride, capacity):
         _check(lib.wsbb_selector_log(self.h, stride, capacity), "selector_log")
@@ -373,6 +381,20 @@ class Engine:
         _check(lib.wsbb_selector_infer(self.h, _p(feat, _f32p), N, _p(scores, _f32p),
                                        _p(tsel, _u32p)), "infer")
         return scores, tsel
+
+    def selector_infer_parents(self, feat, counts):
+        """features [5m][Np] of parents with counts[p] consecutive children:
+        (probs [Np][P], tile masks [ceil(sum counts / 64)][ceil(P/32)])"""
+        feat = np.ascontiguousarray(feat, np.float32)
+        counts = np.ascontiguousarray(counts, np.int32)
+        Np = feat.shape[1]
+        assert counts.shape == (Np,)
+        probs = np.zeros((Np, self.P), np.float32)
+        tsel = np.zeros(((int(counts.sum()) + 63) // 64, (self.P + 31) // 32), np.uint32)
+        _check(lib.wsbb_selector_infer_parents(self.h, _p(feat, _f32p), Np, _p(counts),
+                                               _p(probs, _f32p), _p(tsel, _u32p)),
+               "infer_parents")
+        return probs, tsel
 
     def solve(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
         best = ctypes.c_int32()

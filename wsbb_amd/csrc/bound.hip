@@ -590,7 +590,10 @@ hipError_t launch_pk_src(const DevTables& T, const Src& src, const BoundArgs& A,
 // `scale`.  Lane half*32 + c of the fragment of 32-node block b and feature
 // step t holds features 8t + 4half .. 8t + 4half + 3 of node 32b + c.  One
 // wave per 64-node tile; the remaining work is the row total minus the
-// scheduled jobs, which the head/tail recurrences visit anyway.
+// scheduled jobs, which the head/tail recurrences visit anyway.  The
+// per-machine values pass through the wave's LDS area (rows indexed by
+// constants on the way in); a block table gives, per feature, the one or two
+// rows it sums, so the feature loop has no branches and no register selects.
 template <int MM, int NW, class Src>
 __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs A) {
   extern __shared__ int32_t lds[];
@@ -600,9 +603,22 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
   for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // feature f = g m + k sums rows ra + rb of hv (row 3 MM is zero): one entry
+  // ra | rb << 16 per feature, zero rows for the padding features
+  constexpr int kRows = 3 * MM + 1, kZero = 3 * MM, kFmax = 8 * ((5 * MM + 7) / 8);
+  int32_t* tab = reinterpret_cast<int32_t*>(pl + n * 256);
+  if (threadIdx.x < kFmax) {
+    const int f = threadIdx.x, g = f / m, k = f % m;
+    const int ra = g == 0 ? k : g == 1 ? MM + k : g == 2 || g == 4 ? 2 * MM + k : g == 3 ? k : kZero;
+    const int rb = g == 3 ? 2 * MM + k : g == 4 ? MM + k : kZero;
+    tab[f] = ra | (rb << 16);
+  }
+  __syncthreads();
+  // per wave: head, tail, remaining work and a zero row [3 MM + 1][64]
+  int32_t* hv = tab + kFmax + static_cast<size_t>(wave) * kRows * 64;
+  hv[kZero * 64 + lane] = 0;
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
-  const int F = 5 * m;
   const float sc = 1.0f / A.scale;
   for (int tile0 = blockIdx.x * 4; tile0 < ntiles; tile0 += gridDim.x * 4) {
     const int tile = __builtin_amdgcn_readfirstlane(tile0 + wave);
@@ -658,29 +674,32 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
       q[k] = d2 ? q[k] : ld_const(T.Q0, k);
       sc_[k] = ld_const(T.ptot, k) - sc_[k];
     }
+    // each lane writes and reads back only its own column
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      hv[k * 64 + lane] = r[k];
+      hv[(MM + k) * 64 + lane] = q[k];
+      hv[(2 * MM + k) * 64 + lane] = sc_[k];
+    }
     const int b = node >> 5, c = lane & 31;
-    int g = 0, k = 0;  // feature f = g m + k, uniform
+    const float vs = valid ? sc : 0.f;
+    const int F = 5 * m;
     for (int t = 0; t < A.fsteps; ++t)
+#pragma unroll
       for (int half = 0; half < 2; ++half) {
         uint32_t w[2] = {0u, 0u};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          float x = 0.f;
-          if (g < 5) {
-            const int h = r[k], tl = q[k], rw = sc_[k];
-            const int v = g == 0 ? h : g == 1 ? tl : g == 2 ? rw : g == 3 ? h + rw : rw + tl;
-            x = valid ? static_cast<float>(v) * sc : 0.f;
-            if (++k == m) {
-              k = 0;
-              ++g;
-            }
-          }
+          const int f = 8 * t + 4 * half + i;
+          const int e = tab[f];
+          const float x = static_cast<float>(hv[(e & 0xffff) * 64 + lane] + hv[(e >> 16) * 64 + lane]) * vs;
+          if (A.f32 && valid && f < F) A.f32[static_cast<size_t>(f) * A.ld + node] = x;
           w[i >> 1] |= static_cast<uint32_t>(f2bf_dev(x)) << (16 * (i & 1));
         }
         // lanes 0-31 and 32-63 belong to consecutive 32-node blocks
-        A.frag[(static_cast<size_t>(b) * A.fsteps + t) * 64 + half * 32 + c] = make_uint2(w[0], w[1]);
+        if (A.frag)
+          A.frag[(static_cast<size_t>(b) * A.fsteps + t) * 64 + half * 32 + c] = make_uint2(w[0], w[1]);
       }
-    (void)F;
   }
 }
 
@@ -691,7 +710,8 @@ hipError_t launch_feat_mm(const DevTables& T, const Src& src, const FeatArgs& A,
   int grid = (ntiles + 3) / 4;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * ((MM * T.n + 3) & ~3) + static_cast<size_t>(T.n) * 256;
+  const size_t shmem = sizeof(int32_t) * ((MM * T.n + 3) & ~3) + static_cast<size_t>(T.n) * 256 +
+                       sizeof(int32_t) * (8 * ((5 * MM + 7) / 8) + 4 * (3 * MM + 1) * 64);
   hipLaunchKernelGGL((k_features<MM, NW, Src>), dim3(grid), dim3(256), shmem, stream, T, src, A);
   return hipGetLastError();
 }

@@ -714,11 +714,13 @@ __global__ void k_init(SArgs S, int ub0) {
 
 // Records of the plain solver for the pair selector: every stride-th child
 // of the round (features and LB2-maximising pair) at a slot fixed by its
-// index, so the log does not depend on thread timing.
+// index, so the log does not depend on thread timing.  With desc set the
+// features are the child's parent's (column desc[c] >> 9).
 struct LogArgs {
   const Ctrl* ctrl;
   const float* feat;  // [F][ld]
-  const int* label;   // [ld]
+  const int* label;   // [children]
+  const int* desc;    // child descriptors, or null
   int ld, F, stride, cap;
   float* log_feat;    // [F][cap]
   int* log_label;     // [cap]
@@ -733,8 +735,9 @@ __global__ __launch_bounds__(256) void k_log(LogArgs L) {
     const int slot = base + r;
     if (slot >= L.cap) break;
     const int c = r * L.stride;
+    const int col = L.desc ? (L.desc[c] >> 9) : c;
     for (int f = 0; f < L.F; ++f)
-      L.log_feat[static_cast<size_t>(f) * L.cap + slot] = L.feat[static_cast<size_t>(f) * L.ld + c];
+      L.log_feat[static_cast<size_t>(f) * L.cap + slot] = L.feat[static_cast<size_t>(f) * L.ld + col];
     L.log_label[slot] = L.label[c];
   }
 }
@@ -761,6 +764,7 @@ struct SearchState {
   uint32_t* r_tsel = nullptr;
   uint2* r_xfrag = nullptr;
   int* r_fbl = nullptr;
+  uint16_t* r_prob = nullptr;  // per-parent BF16 probabilities [PCAP][ldp]
   // rounds_per_sync rounds captured once per solve and replayed
   hipGraphExec_t graph = nullptr;
   // optional timing of every bound launch (pairs of events per round)
@@ -779,7 +783,8 @@ static int ensure_aux(wsbb_ctx* ctx) {
   const size_t bl = al(sizeof(int) * static_cast<size_t>(S->CCAP));
   const size_t bt = al(sizeof(uint32_t) * ntiles * nsw);
   const size_t bx = al(sizeof(uint2) * 2 * ntiles * fsteps * 64);
-  hipError_t e = S->aux.reserve(bf + 2 * bl + bt + bx);
+  const size_t bp = al(sizeof(uint16_t) * static_cast<size_t>(S->PCAP) * 32 * ((ctx->T.P + 31) / 32));
+  hipError_t e = S->aux.reserve(bf + 2 * bl + bt + bx + bp);
   if (e != hipSuccess) {
     hip_fail(e, "hipMalloc(selector round buffers)");
     return WSBB_ENOMEM;
@@ -789,6 +794,7 @@ static int ensure_aux(wsbb_ctx* ctx) {
   S->r_fbl = S->aux.as<int>(bf + bl);
   S->r_tsel = S->aux.as<uint32_t>(bf + 2 * bl);
   S->r_xfrag = S->aux.as<uint2>(bf + 2 * bl + bt);
+  S->r_prob = S->aux.as<uint16_t>(bf + 2 * bl + bt + bx);
   return WSBB_OK;
 }
 
@@ -896,23 +902,43 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
   B.pk = ctx->int16_for(A.CCAP);
   hipError_t e;
+  const bool per_parent = selector_parents(ctx);
+  const BatchSource parents{A.p_perm, A.p_l1, A.p_l2, A.PCAP};
+  const float fscale = static_cast<float>(ctx->inst.path_bound);
   if (kind == WSBB_LEARNED) {
-    // BF16 features of every child, MLP pair subset per 64-child tile, LB2
-    // over the subset; children close to the incumbent then get full LB2
-    FeatArgs Fa{A.CCAP, &A.ctrl->n_children, static_cast<float>(ctx->inst.path_bound),
-                selector_fsteps(ctx), SS->r_xfrag};
-    if ((e = launch_features(ctx->T, src, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
-    if ((e = launch_mlp_select(ctx, SS->r_xfrag, A.CCAP, &A.ctrl->n_children, SS->r_tsel,
-                               nullptr, 0)) != hipSuccess)
-      return e;
+    // BF16 features, MLP pair subset per 64-child tile, LB2 over the subset;
+    // children close to the incumbent then get full LB2.  Per parent: the
+    // parents are scored and each tile sums its parents' probabilities.
+    if (per_parent) {
+      FeatArgs Fa{A.PCAP, &A.ctrl->n_parents, fscale, selector_fsteps(ctx), SS->r_xfrag};
+      if ((e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+      if ((e = launch_mlp_select(ctx, SS->r_xfrag, A.PCAP, &A.ctrl->n_parents, nullptr, nullptr, 0,
+                                 nullptr, SS->r_prob)) != hipSuccess)
+        return e;
+      if ((e = launch_parent_tiles(ctx, SS->r_prob, A.p_off, A.c_desc, &A.ctrl->n_children,
+                                   A.CCAP, SS->r_tsel)) != hipSuccess)
+        return e;
+    } else {
+      FeatArgs Fa{A.CCAP, &A.ctrl->n_children, fscale, selector_fsteps(ctx), SS->r_xfrag};
+      if ((e = launch_features(ctx->T, src, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+      if ((e = launch_mlp_select(ctx, SS->r_xfrag, A.CCAP, &A.ctrl->n_children, SS->r_tsel,
+                                 nullptr, 0)) != hipSuccess)
+        return e;
+    }
     B.groups = 1;
     B.tile_sel = SS->r_tsel;
     B.nsw = selector_nsw(ctx);
     B.margin = -1;
   } else if (log_stride > 0) {
-    B.feat = SS->r_feat;
     B.argmax = SS->r_label;
     B.ld = A.CCAP;
+    if (per_parent) {
+      FeatArgs Fa{A.PCAP, &A.ctrl->n_parents, fscale, selector_fsteps(ctx), nullptr,
+                  SS->r_feat, A.PCAP};
+      if ((e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+    } else {
+      B.feat = SS->r_feat;
+    }
   }
   if (ev0) (void)hipEventRecord(ev0, st);
   e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
@@ -939,7 +965,8 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     L.ctrl = A.ctrl;
     L.feat = SS->r_feat;
     L.label = SS->r_label;
-    L.ld = A.CCAP;
+    L.ld = per_parent ? A.PCAP : A.CCAP;
+    L.desc = per_parent ? A.c_desc : nullptr;
     L.F = 5 * ctx->T.m;
     L.stride = log_stride;
     selector_log_buffers(ctx, &L.log_feat, &L.log_label, &L.log_count, &L.cap);

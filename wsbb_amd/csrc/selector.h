@@ -22,9 +22,21 @@ struct MlpTiles {
   const int* cnt;
   int fstride, tstride;
 };
+// With probs set, only the per-node BF16 softmax probabilities are written
+// (node-major rows of selector_ldp(ctx) entries, padding pairs 0).
 hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int* Ndev,
                              uint32_t* tile_sel, float* scores, int ld,
-                             const MlpTiles* tiles = nullptr);
+                             const MlpTiles* tiles = nullptr, uint16_t* probs = nullptr);
+// Pair subsets of 64-child tiles from the probabilities of their parents:
+// tile t covers children [64t, 64t + 64) of the Nc live ones (children of a
+// parent are consecutive, parent p owns [p_off[p], p_off[p+1]), child c's
+// parent is desc[c] >> 9); the mask holds the top-k pairs of the sum over the
+// tile's parents of (children in the tile) x (parent's probabilities).
+hipError_t launch_parent_tiles(wsbb_ctx* ctx, const uint16_t* probs, const int* p_off,
+                               const int* desc, const int* Nc_dev, int Nc_max,
+                               uint32_t* tile_sel);
+int selector_ldp(const wsbb_ctx* ctx);
+bool selector_parents(const wsbb_ctx* ctx);
 int selector_fsteps(const wsbb_ctx* ctx);
 int selector_topk(const wsbb_ctx* ctx);
 int selector_margin(const wsbb_ctx* ctx);

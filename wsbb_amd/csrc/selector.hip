@@ -71,7 +71,47 @@ struct MlpArgs {
   uint32_t* tile_sel;       // [ntiles][nsw]
   float* scores;            // [P][ld] logits, or null
   int ld;
+  uint16_t* probs;          // [N][ldp] BF16 softmax probabilities (OUT == 2)
+  int ldp;                  // multiple of 4, >= 32 PB
 };
+
+// Top-k of a wave's non-negative candidates without sorting.  Candidate
+// (i, lane) has number 64 i + lane; the bit pattern of a non-negative float
+// orders like its value, so the k-th largest value T is built bit by bit
+// from ballot counts (31 rounds of compares and scalar popcounts, no lane
+// exchanges), then sel[i] holds the candidates above T and, of those equal
+// to T, the lowest-numbered ones until k are chosen.
+template <int NI>
+__device__ __forceinline__ void wave_topk(const float (&v)[NI], const bool (&ok)[NI], int k,
+                                          unsigned long long (&sel)[NI]) {
+  uint32_t u[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) u[i] = v[i] > 0.f ? __float_as_uint(v[i]) : 0u;
+  uint32_t T = 0;
+  for (int bit = 30; bit >= 0; --bit) {
+    const uint32_t cand = T | (1u << bit);
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) cnt += __popcll(__ballot(ok[i] && u[i] >= cand));
+    if (cnt >= k) T = cand;
+  }
+  int rem = k;
+  unsigned long long eq[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    sel[i] = __ballot(ok[i] && u[i] > T);
+    rem -= __popcll(sel[i]);
+    eq[i] = __ballot(ok[i] && u[i] == T);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+    while (rem > 0 && eq[i]) {
+      const unsigned long long low = eq[i] & (~eq[i] + 1ull);
+      sel[i] |= low;
+      eq[i] ^= low;
+      --rem;
+    }
+}
 
 __device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
   bf16x4 r;
@@ -86,8 +126,9 @@ __device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
 // eight waves' areas), one 64-node tile per wave and iteration; the weight
 // fragments of both layers are staged in LDS once per block, and each
 // block's feature fragments pass through the wave's LDS area so that every
-// feature load of a block is in flight at once.
-template <int PB, bool SCORES>
+// feature load of a block is in flight at once.  OUT: 0 tile masks, 1 logits
+// and tile masks, 2 per-node probabilities only (node-major BF16 rows).
+template <int PB, int OUT>
 __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   extern __shared__ uint2 s_w[];
   const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
@@ -117,7 +158,8 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
   const int col = lane & 31, half = lane >> 5;
-  const int nwork = A.tile_list ? A.nlist : ntiles;
+  // probabilities only: a work item is one 32-node block
+  const int nwork = A.tile_list ? A.nlist : OUT == 2 ? (N + 31) >> 5 : ntiles;
   for (int w0 = blockIdx.x * nwaves; w0 < nwork; w0 += gridDim.x * nwaves) {
     const int wi = __builtin_amdgcn_readfirstlane(w0 + wave);
     if (wi >= nwork) continue;  // no block barrier below
@@ -126,8 +168,8 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
     float agg[PB];  // tile sums of the probabilities of pairs 32 bp + col
 #pragma unroll
     for (int bp = 0; bp < PB; ++bp) agg[bp] = 0.f;
-    for (int nb = 0; nb < 2; ++nb) {
-      const int blk = tile * 2 + nb;
+    for (int nb = 0; nb < (OUT == 2 ? 1 : 2); ++nb) {
+      const int blk = OUT == 2 ? wi : tile * 2 + nb;
       const int node = blk * 32 + col;
       const bool valid = A.tile_list ? (nb * 32 + col < tcnt) : (node < N);
       // the block's feature fragments go to LDS with every load in flight at
@@ -200,7 +242,7 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
         __builtin_amdgcn_sched_barrier(0);
       }
       // logits; this lane's pairs are 32bp + 8(v/4) + 4half + v%4
-      if constexpr (SCORES) {
+      if constexpr (OUT == 1) {
         int P = A.P;
         asm volatile("" : "+s"(P));  // keep the bounds tests out of the loop preheader
 #pragma unroll
@@ -228,6 +270,25 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
         }
       sum += __shfl_xor(sum, 32, 64);
       const float inv = valid ? 1.f / sum : 0.f;
+      if constexpr (OUT == 2) {
+        // pairs 32bp + 8q + 4half .. +3 of this lane's node: one 8-byte store
+        if (valid) {
+          uint16_t* row = A.probs + static_cast<size_t>(node) * A.ldp + 4 * half;
+#pragma unroll
+          for (int bp = 0; bp < PB; ++bp)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              uint2 w;
+              w.x = f2bf(acc2[bp][4 * q] * inv) |
+                    (static_cast<uint32_t>(f2bf(acc2[bp][4 * q + 1] * inv)) << 16);
+              w.y = f2bf(acc2[bp][4 * q + 2] * inv) |
+                    (static_cast<uint32_t>(f2bf(acc2[bp][4 * q + 3] * inv)) << 16);
+              *reinterpret_cast<uint2*>(row + 32 * bp + 8 * q) = w;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();  // s_x of the next block
+        continue;
+      }
       // tile sums, 32 pairs at a time through this wave's LDS rows (the
       // wave's LDS operations complete in order)
 #pragma unroll
@@ -244,47 +305,90 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
         __builtin_amdgcn_wave_barrier();
       }
     }
+    if constexpr (OUT == 2) continue;
     // top-k pairs of the tile (ties: lower pair index); lanes 0-31 hold the
     // candidates 32 bp + col, padding and the upper half excluded
-    uint32_t mine[PB];
+    bool ok[PB];
 #pragma unroll
-    for (int bp = 0; bp < PB; ++bp) {
-      mine[bp] = 0;
-      if (half == 1 || 32 * bp + col >= A.P) agg[bp] = -1.f;
-    }
-    const int k = min(A.topk, A.P);
-    for (int it = 0; it < k; ++it) {
-      float bv = -2.f;
-      int bs = 1 << 30;
-#pragma unroll
-      for (int bp = 0; bp < PB; ++bp) {
-        const int s = 32 * bp + col;
-        if (!mine[bp] && (agg[bp] > bv || (agg[bp] == bv && s < bs))) {
-          bv = agg[bp];
-          bs = s;
-        }
-      }
-      for (int d = 32; d >= 1; d >>= 1) {
-        const float ov = __shfl_xor(bv, d, 64);
-        const int os = __shfl_xor(bs, d, 64);
-        if (ov > bv || (ov == bv && os < bs)) {
-          bv = ov;
-          bs = os;
-        }
-      }
-#pragma unroll
-      for (int bp = 0; bp < PB; ++bp)
-        if (half == 0 && 32 * bp + col == bs) mine[bp] = 1;
-    }
+    for (int bp = 0; bp < PB; ++bp) ok[bp] = half == 0 && 32 * bp + col < A.P;
+    unsigned long long sel[PB];
+    wave_topk<PB>(agg, ok, min(A.topk, A.P), sel);
     // word bp of the mask: pairs 32 bp .. 32 bp + 31 from lanes 0-31
 #pragma unroll
-    for (int bp = 0; bp < PB; ++bp) {
-      const unsigned long long bal = __ballot(mine[bp] != 0);
+    for (int bp = 0; bp < PB; ++bp)
       if (lane == 0 && bp < A.nsw)
-        A.tile_sel[static_cast<size_t>(tile) * A.tstride + bp] = static_cast<uint32_t>(bal);
-    }
+        A.tile_sel[static_cast<size_t>(tile) * A.tstride + bp] = static_cast<uint32_t>(sel[bp]);
     if (lane == 0)
       for (int w = A.nsw; w < A.tstride; ++w) A.tile_sel[static_cast<size_t>(tile) * A.tstride + w] = 0u;
+  }
+}
+
+// Pair subsets of child tiles from parent-level predictions.  Children of a
+// parent are consecutive, so the parents of a 64-child tile are the range
+// from the first child's parent to the last child's; each adds its
+// probabilities weighted by its number of children in the tile.  One wave per
+// tile, lane l sums pairs 64 i + l; top-k as in the MLP kernel (ties: lower
+// pair index).
+struct ParentTileArgs {
+  const uint16_t* probs;  // [parents][ldp] BF16
+  int ldp, P, nsw, topk;
+  const int* p_off;       // [parents + 1] first child of each parent
+  const int* desc;        // [children] parent << 9 | ...
+  const int* Nc;          // live children
+  uint32_t* tile_sel;     // [tiles][nsw]
+};
+
+template <int NI>
+__global__ __launch_bounds__(256) void k_parent_tiles(ParentTileArgs A) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nc = *A.Nc;
+  const int ntiles = (nc + 63) >> 6;
+  for (int t0 = blockIdx.x * 4; t0 < ntiles; t0 += gridDim.x * 4) {
+    const int tile = __builtin_amdgcn_readfirstlane(t0 + wave);
+    if (tile >= ntiles) continue;
+    const int c0 = tile * 64, c1 = min(c0 + 64, nc);
+    // descriptors and offsets were written by earlier launches: scalar loads
+    const int pa = ld_const(A.desc, c0) >> 9;
+    const int pb = ld_const(A.desc, c1 - 1) >> 9;
+    float agg[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) agg[i] = 0.f;
+    // eight parents per batch: every row load of a batch is in flight at once
+    for (int p0 = pa; p0 <= pb; p0 += 8) {
+      float w[8], x[8][NI];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int p = p0 + j;
+        const bool in = p <= pb;
+        const int lo = in ? max(ld_const(A.p_off, p), c0) : 0;
+        const int hi = in ? min(ld_const(A.p_off, p + 1), c1) : 0;
+        w[j] = static_cast<float>(max(hi - lo, 0));
+        const uint16_t* row = A.probs + static_cast<size_t>(in ? p : pa) * A.ldp;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int s = 64 * i + lane;
+          x[j][i] = s < A.ldp ? bf2f(row[s]) : 0.f;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int i = 0; i < NI; ++i) agg[i] += w[j] * x[j][i];
+    }
+    bool ok[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) ok[i] = 64 * i + lane < A.P;
+    unsigned long long sel[NI];
+    wave_topk<NI>(agg, ok, min(A.topk, A.P), sel);
+    // candidates 64 i .. 64 i + 63 = mask words 2i and 2i + 1
+    if (lane == 0) {
+      uint32_t* ts = A.tile_sel + static_cast<size_t>(tile) * A.nsw;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        if (2 * i < A.nsw) ts[2 * i] = static_cast<uint32_t>(sel[i]);
+        if (2 * i + 1 < A.nsw) ts[2 * i + 1] = static_cast<uint32_t>(sel[i] >> 32);
+      }
+    }
   }
 }
 
@@ -472,6 +576,10 @@ __global__ void k_pad_b2(const float* b2, float* b2p, int P, int Pp) {
 struct SelectorState {
   int F = 0, Fsteps = 0, P = 0, PB = 0, nsw = 0;
   int topk = 16, margin = 0;
+  // the network predicts, from a parent's features, the pairs that maximise
+  // LB2 over its children (logging records the parent's features with each
+  // child's label; learned rounds score parents, not children)
+  bool parents = false;
   // parameters: masters (W1, b1, W2, b2), Adam moments, gradients
   DevBuf params, frags;
   float *W1, *b1, *W2, *b2, *mW1, *mb1, *mW2, *mb2, *vW1, *vb1, *vW2, *vb2;
@@ -562,7 +670,8 @@ static hipError_t refresh_frags(wsbb_ctx* ctx) {
 }
 
 hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int* Ndev,
-                             uint32_t* tile_sel, float* scores, int ld, const MlpTiles* tl) {
+                             uint32_t* tile_sel, float* scores, int ld, const MlpTiles* tl,
+                             uint16_t* probs) {
   SelectorState* S = ctx->selector;
   hipError_t e = refresh_frags(ctx);
   if (e != hipSuccess) return e;
@@ -586,22 +695,30 @@ hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int
   A.tile_sel = tile_sel;
   A.scores = scores;
   A.ld = ld;
-  const int ntiles = tl ? tl->nlist : (N + 63) / 64;
+  A.probs = probs;
+  A.ldp = 32 * S->PB;
+  const int ntiles = tl ? tl->nlist : probs ? (N + 31) / 32 : (N + 63) / 64;
   if (ntiles <= 0) return hipSuccess;
   // one persistent block per CU: the weights are staged once per block, and
-  // eight waves (two per SIMD) share them when the LDS holds their areas
+  // eight waves (two per SIMD) share them when the LDS holds their areas and
+  // there is work for two waves per SIMD
   A.wstride = std::max(S->Fsteps * 64, (32 * 33 + 1) / 2);
   const size_t wbytes = sizeof(uint2) * (4 * S->Fsteps * 64 + S->PB * 16 * 64);
-  const int waves = wbytes + sizeof(uint2) * 8 * A.wstride <= 160 * 1024 ? 8 : 4;
+  const int waves = wbytes + sizeof(uint2) * 8 * A.wstride <= 160 * 1024 &&
+                            ntiles >= 8 * ctx->num_cus
+                        ? 8
+                        : 4;
   const int grid = std::max(1, std::min((ntiles + waves - 1) / waves, ctx->num_cus));
   const size_t shmem = wbytes + sizeof(uint2) * waves * A.wstride;
   switch (S->PB) {
 #define WSBB_PB(v)                                                                          \
   case v:                                                                                   \
-    if (scores)                                                                             \
-      hipLaunchKernelGGL((k_mlp_select<v, true>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
+    if (probs)                                                                              \
+      hipLaunchKernelGGL((k_mlp_select<v, 2>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
+    else if (scores)                                                                        \
+      hipLaunchKernelGGL((k_mlp_select<v, 1>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
     else                                                                                    \
-      hipLaunchKernelGGL((k_mlp_select<v, false>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
+      hipLaunchKernelGGL((k_mlp_select<v, 0>), dim3(grid), dim3(64 * waves), shmem, ctx->stream, A); \
     break;
     WSBB_PB(1) WSBB_PB(2) WSBB_PB(3) WSBB_PB(4) WSBB_PB(5) WSBB_PB(6) WSBB_PB(7) WSBB_PB(8)
     WSBB_PB(9)
@@ -611,7 +728,29 @@ hipError_t launch_mlp_select(wsbb_ctx* ctx, const uint2* xfrag, int N, const int
   return hipGetLastError();
 }
 
+hipError_t launch_parent_tiles(wsbb_ctx* ctx, const uint16_t* probs, const int* p_off,
+                               const int* desc, const int* Nc_dev, int Nc_max,
+                               uint32_t* tile_sel) {
+  SelectorState* S = ctx->selector;
+  ParentTileArgs A{probs, 32 * S->PB, S->P, S->nsw, S->topk, p_off, desc, Nc_dev, tile_sel};
+  const int ntiles = (Nc_max + 63) / 64;
+  if (ntiles <= 0) return hipSuccess;
+  const int grid = std::max(1, std::min((ntiles + 3) / 4, ctx->grid_cap()));
+  switch ((S->P + 63) / 64) {
+#define WSBB_NI(v)                                                                         \
+  case v:                                                                                  \
+    hipLaunchKernelGGL((k_parent_tiles<v>), dim3(grid), dim3(256), 0, ctx->stream, A);   \
+    break;
+    WSBB_NI(1) WSBB_NI(2) WSBB_NI(3) WSBB_NI(4) WSBB_NI(5)
+#undef WSBB_NI
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 int selector_fsteps(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->Fsteps : 0; }
+int selector_ldp(const wsbb_ctx* ctx) { return ctx->selector ? 32 * ctx->selector->PB : 0; }
+bool selector_parents(const wsbb_ctx* ctx) { return ctx->selector && ctx->selector->parents; }
 
 int selector_topk(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->topk : 0; }
 int selector_margin(const wsbb_ctx* ctx) { return ctx->selector ? ctx->selector->margin : 0; }
@@ -759,6 +898,16 @@ int wsbb_selector_config(wsbb_ctx* ctx, int topk, int margin) {
   }
   ctx->selector->topk = topk;
   ctx->selector->margin = margin;
+  return WSBB_OK;
+}
+
+int wsbb_selector_parents(wsbb_ctx* ctx, int on) {
+  SEL_CHECK(ctx);
+  if (on != 0 && on != 1) {
+    set_error("parents must be 0 or 1");
+    return WSBB_EINVAL;
+  }
+  ctx->selector->parents = on != 0;
   return WSBB_OK;
 }
 
@@ -936,6 +1085,62 @@ int wsbb_selector_infer(wsbb_ctx* ctx, const float* feat, int N, float* scores, 
   if (scores) SEL_HIP(hipMemcpyAsync(scores, dsc, sb, hipMemcpyDeviceToHost, ctx->stream), "D2H scores");
   if (tile_sel) SEL_HIP(hipMemcpyAsync(tile_sel, dts, tb, hipMemcpyDeviceToHost, ctx->stream), "D2H sel");
   SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  return WSBB_OK;
+}
+
+int wsbb_selector_infer_parents(wsbb_ctx* ctx, const float* feat, int Np, const int32_t* counts,
+                                float* probs, uint32_t* tile_sel) {
+  SEL_CHECK(ctx);
+  SelectorState* S = ctx->selector;
+  if (Np <= 0 || !feat || !counts) return WSBB_EINVAL;
+  std::vector<int> off(Np + 1, 0);
+  for (int p = 0; p < Np; ++p) {
+    if (counts[p] < 0) return WSBB_EINVAL;
+    off[p + 1] = off[p] + counts[p];
+  }
+  const int Nc = off[Np];
+  if (Nc <= 0 || Nc > (1 << 30)) return WSBB_EINVAL;
+  std::vector<int> desc(Nc);
+  for (int p = 0; p < Np; ++p)
+    for (int c = off[p]; c < off[p + 1]; ++c) desc[c] = p << 9;
+  const int ldp = 32 * S->PB, ntiles = (Nc + 63) / 64;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t fb = al(sizeof(float) * static_cast<size_t>(S->F) * Np);
+  const size_t pb = al(sizeof(uint16_t) * static_cast<size_t>(Np) * ldp);
+  const size_t ob = al(sizeof(int) * (static_cast<size_t>(Np) + 2));
+  const size_t db = al(sizeof(int) * static_cast<size_t>(Nc));
+  const size_t tb = al(sizeof(uint32_t) * static_cast<size_t>(ntiles) * S->nsw);
+  SEL_HIP(S->io.reserve(fb + pb + ob + db + tb), "hipMalloc(io)");
+  float* dfeat = S->io.as<float>();
+  uint16_t* dprob = S->io.as<uint16_t>(fb);
+  int* doff = S->io.as<int>(fb + pb);
+  int* ddesc = S->io.as<int>(fb + pb + ob);
+  uint32_t* dts = S->io.as<uint32_t>(fb + pb + ob + db);
+  off.push_back(Nc);  // live child count after the offsets
+  SEL_HIP(hipMemcpyAsync(dfeat, feat, sizeof(float) * S->F * static_cast<size_t>(Np),
+                         hipMemcpyHostToDevice, ctx->stream), "H2D feat");
+  SEL_HIP(hipMemcpyAsync(doff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, ctx->stream),
+          "H2D offsets");
+  SEL_HIP(hipMemcpyAsync(ddesc, desc.data(), sizeof(int) * Nc, hipMemcpyHostToDevice, ctx->stream),
+          "H2D desc");
+  const size_t xb = sizeof(uint2) * static_cast<size_t>(2 * ((Np + 63) / 64)) * S->Fsteps * 64;
+  SEL_HIP(S->xio.reserve(xb), "hipMalloc(feature fragments)");
+  uint2* dx = S->xio.as<uint2>();
+  hipLaunchKernelGGL(k_feat_pack, dim3(std::min<long long>((xb / 8 + 255) / 256, 4096)), dim3(256), 0,
+                     ctx->stream, dfeat, Np, Np, S->F, S->Fsteps, dx);
+  SEL_HIP(hipGetLastError(), "pack launch");
+  SEL_HIP(launch_mlp_select(ctx, dx, Np, nullptr, nullptr, nullptr, 0, nullptr, dprob), "mlp launch");
+  SEL_HIP(launch_parent_tiles(ctx, dprob, doff, ddesc, doff + Np + 1, Nc, dts), "tiles launch");
+  std::vector<uint16_t> hp(static_cast<size_t>(Np) * ldp);
+  SEL_HIP(hipMemcpyAsync(hp.data(), dprob, sizeof(uint16_t) * hp.size(), hipMemcpyDeviceToHost, ctx->stream),
+          "D2H probs");
+  if (tile_sel)
+    SEL_HIP(hipMemcpyAsync(tile_sel, dts, sizeof(uint32_t) * static_cast<size_t>(ntiles) * S->nsw,
+                           hipMemcpyDeviceToHost, ctx->stream), "D2H sel");
+  SEL_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  if (probs)
+    for (int p = 0; p < Np; ++p)
+      for (int s = 0; s < S->P; ++s) probs[static_cast<size_t>(p) * S->P + s] = bf2f(hp[static_cast<size_t>(p) * ldp + s]);
   return WSBB_OK;
 }
 
