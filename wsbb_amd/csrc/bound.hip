@@ -590,24 +590,32 @@ hipError_t launch_pk_src(const DevTables& T, const Src& src, const BoundArgs& A,
 // `scale`.  Lane half*32 + c of the fragment of 32-node block b and feature
 // step t holds features 8t + 4half .. 8t + 4half + 3 of node 32b + c.  One
 // wave per 64-node tile; the remaining work is the row total minus the
-// scheduled jobs, which the head/tail recurrences visit anyway.  The
-// per-machine values pass through the wave's LDS area (rows indexed by
-// constants on the way in); a block table gives, per feature, the one or two
-// rows it sums, so the feature loop has no branches and no register selects.
-template <int MM, int NW, class Src>
+// scheduled jobs, which the head/tail recurrences visit anyway.
+//
+// A launch over a round's parents is small (a few hundred waves), so the
+// kernel is written for latency: processing times sit in LDS job-major (a
+// job's machines are one 16-byte read per four), the next job index is read
+// while the current one is processed, and with EXACT (m == MM) the feature
+// loop is unrolled over constant indices.  Otherwise the per-machine values
+// pass through the wave's LDS area and a block table gives, per feature, the
+// one or two rows it sums, so that loop has no branches and no selects.
+template <int MM, bool EXACT, class Src>
 __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs A) {
   extern __shared__ int32_t lds[];
+  static_assert(MM % 4 == 0, "machine rows are read four at a time");
   const int n = T.n, m = T.m;
-  int32_t* sp = lds;  // [MM][n], rows m..MM-1 zero
-  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + ((MM * n + 3) & ~3));  // [n][256]
-  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
-  __syncthreads();
+  int32_t* sp = lds;  // [n][MM] job-major, machines m..MM-1 zero
+  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * n);  // [n][256]
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) {
+    const int j = i / MM, k = i - j * MM;
+    sp[i] = k < m ? T.p[k * n + j] : 0;
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // feature f = g m + k sums rows ra + rb of hv (row 3 MM is zero): one entry
   // ra | rb << 16 per feature, zero rows for the padding features
   constexpr int kRows = 3 * MM + 1, kZero = 3 * MM, kFmax = 8 * ((5 * MM + 7) / 8);
   int32_t* tab = reinterpret_cast<int32_t*>(pl + n * 256);
-  if (threadIdx.x < kFmax) {
+  if (!EXACT && threadIdx.x < kFmax) {
     const int f = threadIdx.x, g = f / m, k = f % m;
     const int ra = g == 0 ? k : g == 1 ? MM + k : g == 2 || g == 4 ? 2 * MM + k : g == 3 ? k : kZero;
     const int rb = g == 3 ? 2 * MM + k : g == 4 ? MM + k : kZero;
@@ -616,7 +624,7 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
   __syncthreads();
   // per wave: head, tail, remaining work and a zero row [3 MM + 1][64]
   int32_t* hv = tab + kFmax + static_cast<size_t>(wave) * kRows * 64;
-  hv[kZero * 64 + lane] = 0;
+  if (!EXACT) hv[kZero * 64 + lane] = 0;
   const int N = A.Ndev ? *A.Ndev : A.N;
   const int ntiles = (N + 63) >> 6;
   const float sc = 1.0f / A.scale;
@@ -633,13 +641,13 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
     }
     const int d1 = l1 + 1, d2 = n - l2;
     if (valid) {
-      for (int i0 = 0; i0 < n; i0 += 16) {
-        uint32_t v[16];
+      for (int i0 = 0; i0 < n; i0 += 32) {
+        uint32_t v[32];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < 32; ++u)
           if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < 32; ++u)
           if (i0 + u < n) pl[(i0 + u) * 256 + threadIdx.x] = static_cast<uint8_t>(v[u]);
       }
     }
@@ -647,24 +655,41 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
     int r[MM], q[MM], sc_[MM];
 #pragma unroll
     for (int k = 0; k < MM; ++k) r[k] = q[k] = sc_[k] = 0;
+    auto row = [&](int j, int (&pk)[MM]) {
+      const int4* rp = reinterpret_cast<const int4*>(sp + j * MM);
+#pragma unroll
+      for (int k4 = 0; k4 < MM / 4; ++k4) {
+        const int4 v = rp[k4];
+        pk[4 * k4] = v.x;
+        pk[4 * k4 + 1] = v.y;
+        pk[4 * k4 + 2] = v.z;
+        pk[4 * k4 + 3] = v.w;
+      }
+    };
+    int jn = d1 > 0 ? perm_at(mine, 256, sw0, sw1, 0) : 0;
     for (int i = 0; i < d1; ++i) {
-      const int j = perm_at(mine, 256, sw0, sw1, i);
-      r[0] += sp[j];
-      sc_[0] += sp[j];
+      const int j = jn;
+      if (i + 1 < d1) jn = perm_at(mine, 256, sw0, sw1, i + 1);
+      int pk[MM];
+      row(j, pk);
+      r[0] += pk[0];
+      sc_[0] += pk[0];
 #pragma unroll
       for (int k = 1; k < MM; ++k) {
-        const int pk = sp[k * n + j];
-        r[k] = max(r[k], r[k - 1]) + pk;
-        sc_[k] += pk;
+        r[k] = max(r[k], r[k - 1]) + pk[k];
+        sc_[k] += pk[k];
       }
     }
+    jn = d2 > 0 ? perm_at(mine, 256, sw0, sw1, n - 1) : 0;
     for (int i = n - 1; i >= l2; --i) {
-      const int j = perm_at(mine, 256, sw0, sw1, i);
+      const int j = jn;
+      if (i - 1 >= l2) jn = perm_at(mine, 256, sw0, sw1, i - 1);
+      int pk[MM];
+      row(j, pk);
 #pragma unroll
       for (int k = MM - 1; k >= 0; --k) {
-        const int pk = sp[k * n + j];
-        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk;
-        sc_[k] += pk;
+        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
+        sc_[k] += pk[k];
       }
     }
     // per machine: head, tail, remaining work
@@ -674,45 +699,78 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
       q[k] = d2 ? q[k] : ld_const(T.Q0, k);
       sc_[k] = ld_const(T.ptot, k) - sc_[k];
     }
-    // each lane writes and reads back only its own column
-#pragma unroll
-    for (int k = 0; k < MM; ++k) {
-      hv[k * 64 + lane] = r[k];
-      hv[(MM + k) * 64 + lane] = q[k];
-      hv[(2 * MM + k) * 64 + lane] = sc_[k];
-    }
     const int b = node >> 5, c = lane & 31;
     const float vs = valid ? sc : 0.f;
-    const int F = 5 * m;
-    for (int t = 0; t < A.fsteps; ++t)
+    if constexpr (EXACT) {
+      // the FP32 copy (logging only) is a separate instance of the loop, so
+      // the common case carries no per-feature test
+      auto emit = [&](auto with_f32) {
+        float* fo = A.f32 + node;  // advanced by ld per feature
+        uint2* fr = A.frag + static_cast<size_t>(b) * A.fsteps * 64 + c;
 #pragma unroll
-      for (int half = 0; half < 2; ++half) {
-        uint32_t w[2] = {0u, 0u};
+        for (int t = 0; t < (5 * MM + 7) / 8; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int f = 8 * t + 4 * half + i;
-          const int e = tab[f];
-          const float x = static_cast<float>(hv[(e & 0xffff) * 64 + lane] + hv[(e >> 16) * 64 + lane]) * vs;
-          if (A.f32 && valid && f < F) A.f32[static_cast<size_t>(f) * A.ld + node] = x;
-          w[i >> 1] |= static_cast<uint32_t>(f2bf_dev(x)) << (16 * (i & 1));
-        }
-        // lanes 0-31 and 32-63 belong to consecutive 32-node blocks
-        if (A.frag)
-          A.frag[(static_cast<size_t>(b) * A.fsteps + t) * 64 + half * 32 + c] = make_uint2(w[0], w[1]);
+          for (int half = 0; half < 2; ++half) {
+            uint32_t w[2] = {0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int f = 8 * t + 4 * half + i, g = f / MM, k = f % MM;
+              if (f < 5 * MM) {
+                const int v = g == 0 ? r[k] : g == 1 ? q[k] : g == 2 ? sc_[k] : g == 3 ? r[k] + sc_[k] : sc_[k] + q[k];
+                const float x = static_cast<float>(v) * vs;
+                if constexpr (decltype(with_f32)::value) {
+                  if (valid) *fo = x;
+                  fo += A.ld;
+                }
+                w[i >> 1] |= static_cast<uint32_t>(f2bf_dev(x)) << (16 * (i & 1));
+              }
+            }
+            // lanes 0-31 and 32-63 belong to consecutive 32-node blocks
+            if (A.frag) fr[t * 64 + half * 32] = make_uint2(w[0], w[1]);
+          }
+      };
+      if (A.f32)
+        emit(std::true_type{});
+      else
+        emit(std::false_type{});
+    } else {
+      // each lane writes and reads back only its own column
+#pragma unroll
+      for (int k = 0; k < MM; ++k) {
+        hv[k * 64 + lane] = r[k];
+        hv[(MM + k) * 64 + lane] = q[k];
+        hv[(2 * MM + k) * 64 + lane] = sc_[k];
       }
+      const int F = 5 * m;
+      for (int t = 0; t < A.fsteps; ++t)
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          uint32_t w[2] = {0u, 0u};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 8 * t + 4 * half + i;
+            const int e = tab[f];
+            const float x = static_cast<float>(hv[(e & 0xffff) * 64 + lane] + hv[(e >> 16) * 64 + lane]) * vs;
+            if (A.f32 && valid && f < F) A.f32[static_cast<size_t>(f) * A.ld + node] = x;
+            w[i >> 1] |= static_cast<uint32_t>(f2bf_dev(x)) << (16 * (i & 1));
+          }
+          if (A.frag)
+            A.frag[(static_cast<size_t>(b) * A.fsteps + t) * 64 + half * 32 + c] = make_uint2(w[0], w[1]);
+        }
+    }
   }
 }
 
-template <int MM, int NW, class Src>
+template <int MM, bool EXACT, class Src>
 hipError_t launch_feat_mm(const DevTables& T, const Src& src, const FeatArgs& A, int grid_cap,
                           hipStream_t stream) {
   const int ntiles = (A.N + 63) / 64;
   int grid = (ntiles + 3) / 4;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * ((MM * T.n + 3) & ~3) + static_cast<size_t>(T.n) * 256 +
-                       sizeof(int32_t) * (8 * ((5 * MM + 7) / 8) + 4 * (3 * MM + 1) * 64);
-  hipLaunchKernelGGL((k_features<MM, NW, Src>), dim3(grid), dim3(256), shmem, stream, T, src, A);
+  const size_t shmem = sizeof(int32_t) * MM * T.n + static_cast<size_t>(T.n) * 256 +
+                       (EXACT ? 0 : sizeof(int32_t) * (8 * ((5 * MM + 7) / 8) + 4 * (3 * MM + 1) * 64));
+  hipLaunchKernelGGL((k_features<MM, EXACT, Src>), dim3(grid), dim3(256), shmem, stream, T, src, A);
   return hipGetLastError();
 }
 
@@ -720,13 +778,22 @@ template <class Src>
 hipError_t launch_feat_src(const DevTables& T, const Src& src, const FeatArgs& A, int grid_cap,
                            hipStream_t stream) {
   if (A.N <= 0) return hipSuccess;
+  if (A.fsteps != (5 * T.m + 7) / 8) return hipErrorInvalidValue;
+  switch (T.m) {
+    case 8: return launch_feat_mm<8, true, Src>(T, src, A, grid_cap, stream);
+    case 12: return launch_feat_mm<12, true, Src>(T, src, A, grid_cap, stream);
+    case 16: return launch_feat_mm<16, true, Src>(T, src, A, grid_cap, stream);
+    case 20: return launch_feat_mm<20, true, Src>(T, src, A, grid_cap, stream);
+    case 24: return launch_feat_mm<24, true, Src>(T, src, A, grid_cap, stream);
+    default: break;
+  }
   const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
   switch (mm) {
-    case 8: return launch_feat_mm<8, 1, Src>(T, src, A, grid_cap, stream);
-    case 12: return launch_feat_mm<12, 1, Src>(T, src, A, grid_cap, stream);
-    case 16: return launch_feat_mm<16, 1, Src>(T, src, A, grid_cap, stream);
-    case 20: return launch_feat_mm<20, 1, Src>(T, src, A, grid_cap, stream);
-    default: return launch_feat_mm<24, 1, Src>(T, src, A, grid_cap, stream);
+    case 8: return launch_feat_mm<8, false, Src>(T, src, A, grid_cap, stream);
+    case 12: return launch_feat_mm<12, false, Src>(T, src, A, grid_cap, stream);
+    case 16: return launch_feat_mm<16, false, Src>(T, src, A, grid_cap, stream);
+    case 20: return launch_feat_mm<20, false, Src>(T, src, A, grid_cap, stream);
+    default: return launch_feat_mm<24, false, Src>(T, src, A, grid_cap, stream);
   }
 }
 

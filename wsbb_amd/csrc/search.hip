@@ -326,64 +326,46 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
 
 // ---- expand: parents out of the stack, child descriptors ------------------
 // A thread per parent copies its permutation column (consecutive parents sit
-// at consecutive stack slots, so every position row is read coalesced); the
-// wave then writes the child descriptors of its 64 parents one parent at a
-// time, lanes over consecutive children.
+// at consecutive stack slots, so every position row is read coalesced) and
+// writes its children's descriptors.  Written for latency (a round has a few
+// thousand parents): one batch of loads per column chunk, no lane exchanges.
 __global__ __launch_bounds__(256) void k_expand(SArgs S) {
   const Ctrl* C = S.ctrl;
   const int np = C->n_parents;
   const int n = S.n, m = S.m, P = S.P;
   const int lane = threadIdx.x & 63;
   long long work = 0;
-  for (int pb = blockIdx.x * blockDim.x; pb < np; pb += gridDim.x * blockDim.x) {
-    const int p = pb + threadIdx.x;
-    int l1 = 0, l2 = 0, off = 0;
-    if (p < np) {
-      const int si = S.p_sel[p];
-      // loads of a chunk all issue before its stores (the arrays may alias
-      // as far as the compiler knows)
-      for (int i0 = 0; i0 < n; i0 += 16) {
-        uint8_t v[16];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    const int si = S.p_sel[p];
+    const int l1 = S.s_l1[si], l2 = S.s_l2[si], cost = S.s_cost[si], off = S.p_off[p];
+    // loads of a chunk all issue before its stores (the arrays may alias as
+    // far as the compiler knows)
+    for (int i0 = 0; i0 < n; i0 += 32) {
+      uint8_t v[32];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-          if (i0 + u < n) v[u] = S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + si];
+      for (int u = 0; u < 32; ++u)
+        if (i0 + u < n) v[u] = S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + si];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-          if (i0 + u < n) S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
-      }
-      l1 = S.s_l1[si];
-      l2 = S.s_l2[si];
-      S.p_l1[p] = l1;
-      S.p_l2[p] = l2;
-      S.p_cost[p] = S.s_cost[si];
-      off = S.p_off[p];
-      // algorithmic bound work of the children: per child with prefix d1,
-      // suffix d2 and u free jobs, P(n + 4u + 4) + 2m d1 + m(m+1) d2 int ops
-      const long long d1 = l1 + 1, d2 = n - l2, u = l2 - l1 - 1;
-      const long long mm1 = static_cast<long long>(m) * (m + 1);
-      if (d1 + d2 == n - 2) {
-        work += 2 * (P * (n + 4LL) + 2LL * m * (d1 + 2) + mm1 * d2);
-      } else {
-        const long long base = P * (n + 4 * (u - 1) + 4);
-        work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
-      }
+      for (int u = 0; u < 32; ++u)
+        if (i0 + u < n) S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
     }
-    const int wbase = pb + (threadIdx.x & ~63);
-    const int cntw = min(64, np - wbase);  // wave-uniform
-    for (int q = 0; q < cntw; ++q) {
-      // q is wave-uniform: read lane q's values into scalar registers
-      const int ql1 = __builtin_amdgcn_readlane(l1, q), ql2 = __builtin_amdgcn_readlane(l2, q),
-                qoff = __builtin_amdgcn_readlane(off, q);
-      const int tag = (wbase + q) << 9;
-      const int u = ql2 - ql1 - 1;
-      if ((ql1 + 1) + (n - ql2) == n - 2) {  // simple parent: two leaves, BEGIN side
-        if (lane < 2) S.c_desc[qoff + lane] = tag | (ql1 + 1 + lane);
-      } else {                               // BEGIN children then END children
-        for (int c = lane; c < 2 * u; c += 64) {
-          const int side = c >= u;
-          S.c_desc[qoff + c] = tag | (side << 8) | (ql1 + 1 + c - side * u);
-        }
-      }
+    S.p_l1[p] = l1;
+    S.p_l2[p] = l2;
+    S.p_cost[p] = cost;
+    const int tag = p << 9, u = l2 - l1 - 1;
+    // algorithmic bound work of the children: per child with prefix d1,
+    // suffix d2 and u free jobs, P(n + 4u + 4) + 2m d1 + m(m+1) d2 int ops
+    const long long d1 = l1 + 1, d2 = n - l2;
+    const long long mm1 = static_cast<long long>(m) * (m + 1);
+    if (d1 + d2 == n - 2) {  // simple parent: two leaves, BEGIN side
+      S.c_desc[off] = tag | (l1 + 1);
+      S.c_desc[off + 1] = tag | (l1 + 2);
+      work += 2 * (P * (n + 4LL) + 2LL * m * (d1 + 2) + mm1 * d2);
+    } else {                 // BEGIN children then END children
+      for (int c = 0; c < u; ++c) S.c_desc[off + c] = tag | (l1 + 1 + c);
+      for (int c = 0; c < u; ++c) S.c_desc[off + u + c] = tag | (1 << 8) | (l1 + 1 + c);
+      const long long base = P * (n + 4 * (u - 1LL) + 4);
+      work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
     }
   }
   for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
