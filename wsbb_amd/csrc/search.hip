@@ -49,7 +49,9 @@ enum : int {
   kStImproved, kStMaxPool, kStRounds, kStSubset, kStFallback, kStPairs, kStWork
 };
 constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
-constexpr int kSelWin = 4096;  // candidates per select window (4 per thread)
+constexpr int kSelWin = 16384;  // candidates per select window after the first
+constexpr int kSelWin0 = 4096;  // candidates in the first window
+constexpr int kSettleK = 16;   // consecutive parents per settle thread
 
 struct Ctrl {
   int status;
@@ -105,15 +107,32 @@ struct OpMin {
   __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
 
+// Wave-wide scans on DPP lane moves: row shifts by 1, 2, 4, 8 scan each
+// 16-lane row, the row broadcasts of lanes 15 and 31 carry the row totals
+// upward (GFX9 DPP controls 0x111-0x118, 0x142, 0x143; wave shift right by
+// one is 0x138).  Each step is a VALU move, not an LDS round trip.  Every
+// lane of the wave must be active.
+#define WSBB_DPP(v, ident, ctrl, rmask) __builtin_amdgcn_update_dpp((ident), (v), (ctrl), (rmask), 0xf, false)
 template <class Op>
-__device__ __forceinline__ int wave_incl_scan(int v, int lane, int ident, Op op) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(v, d, 64);
-    v = op(v, lane >= d ? o : ident);
-  }
+__device__ __forceinline__ int wave_incl_scan(int v, int ident, Op op) {
+  v = op(v, WSBB_DPP(v, ident, 0x111, 0xf));
+  v = op(v, WSBB_DPP(v, ident, 0x112, 0xf));
+  v = op(v, WSBB_DPP(v, ident, 0x114, 0xf));
+  v = op(v, WSBB_DPP(v, ident, 0x118, 0xf));
+  v = op(v, WSBB_DPP(v, ident, 0x142, 0xa));
+  v = op(v, WSBB_DPP(v, ident, 0x143, 0xc));
   return v;
 }
+// value of the lane below (ident in lane 0)
+__device__ __forceinline__ int wave_shr1(int v, int ident) { return WSBB_DPP(v, ident, 0x138, 0xf); }
+// reduction over the wave, in every lane (wave-uniform)
+template <class Op>
+__device__ __forceinline__ int wave_reduce(int v, int ident, Op op) {
+  return __builtin_amdgcn_readlane(wave_incl_scan(v, ident, op), 63);
+}
+struct OpMax {
+  __device__ int operator()(int a, int b) const { return a > b ? a : b; }
+};
 
 // exclusive scan of one value per thread over a 1024-thread block; *total
 // receives the reduction of all values (every thread)
@@ -121,19 +140,19 @@ template <class Op>
 __device__ __forceinline__ int block_exscan1024(int v, int ident, Op op, int* s_w16,
                                                 int* total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int incl = wave_incl_scan(v, lane, ident, op);
-  const int excl_w = __shfl_up(incl, 1, 64);
+  const int incl = wave_incl_scan(v, ident, op);
+  const int excl_w = wave_shr1(incl, ident);
   if (lane == 63) s_w16[wave] = incl;
   __syncthreads();
   if (wave == 0) {
     const int x = lane < 16 ? s_w16[lane] : ident;
-    const int xi = wave_incl_scan(x, lane, ident, op);
-    const int xe = __shfl_up(xi, 1, 64);
-    if (lane < 16) s_w16[16 + lane] = lane == 0 ? ident : xe;  // exclusive wave prefixes
+    const int xi = wave_incl_scan(x, ident, op);
+    const int xe = wave_shr1(xi, ident);
+    if (lane < 16) s_w16[16 + lane] = xe;  // exclusive wave prefixes
     if (lane == 15) s_w16[32] = xi;
   }
   __syncthreads();
-  const int res = op(s_w16[16 + wave], lane == 0 ? ident : excl_w);
+  const int res = op(s_w16[16 + wave], excl_w);
   *total = s_w16[32];
   __syncthreads();
   return res;
@@ -143,23 +162,31 @@ __device__ __forceinline__ int block_exscan1024(int v, int* s_w16, int* total) {
 }
 
 // first index i in [lo, hi) with a[i] >= thr for nondecreasing a (hi if
-// none), by the whole wave: one probe per lane narrows to 64 candidates,
-// a second pass picks the first
+// none), by the whole wave: each round probes the last element of 64 equal
+// buckets and keeps the first bucket that reaches thr, until at most 64
+// candidates remain; a last ballot picks the first
+// LDS index of candidate i in the select window: one pad word per 16
+// entries keeps a thread's 16 consecutive candidates out of its neighbours'
+// banks
+__device__ __forceinline__ int sel_pad(int i) { return i + (i >> 4); }
+
 __device__ __forceinline__ int wave_first_at_least(const int* a, int lo, int hi, int thr,
                                                    int lane) {
-  const int L = hi - lo;
-  if (L <= 0) return hi;
-  const int stride = (L + 63) >> 6;
-  int idx = lo + (lane + 1) * stride - 1;
-  idx = idx < hi - 1 ? idx : hi - 1;
-  const unsigned long long b1 = __ballot(a[idx] >= thr);
-  if (b1 == 0) return hi;
-  const int f = __ffsll(static_cast<long long>(b1)) - 1;
-  const int base = lo + f * stride;
-  const int j = base + lane;
-  const bool ok = j < hi && j < base + stride && a[j] >= thr;
-  const unsigned long long b2 = __ballot(ok);
-  return base + __ffsll(static_cast<long long>(b2)) - 1;
+  const int hi0 = hi;
+  if (hi - lo <= 0) return hi0;
+  while (hi - lo > 64) {
+    const int stride = (hi - lo + 63) >> 6;
+    int idx = lo + (lane + 1) * stride - 1;
+    idx = idx < hi - 1 ? idx : hi - 1;
+    const unsigned long long b1 = __ballot(a[sel_pad(idx)] >= thr);
+    if (b1 == 0) return hi0;
+    const int f = __ffsll(static_cast<long long>(b1)) - 1;
+    lo += f * stride;
+    hi = min(lo + stride, hi);
+  }
+  const int j = lo + lane;
+  const unsigned long long b2 = __ballot(j < hi && a[sel_pad(j)] >= thr);
+  return b2 ? lo + __ffsll(static_cast<long long>(b2)) - 1 : hi0;
 }
 
 // ---- select: one block ------------------------------------------------------
@@ -171,14 +198,113 @@ __device__ __forceinline__ int wave_first_at_least(const int* a, int lo, int hi,
 // which is the parent push_back1 turns away and the next sub-batch starts
 // with.  Wave 0 walks the boundaries with wave-wide searches; the taken
 // parents and their first-child offsets are then written by all threads.
+// One window of KPER * 1024 candidates; the carried state is block-uniform.
+struct SelState {
+  int carry, nb, np, nc, pruned, consumed, stop;
+};
+
+template <int KPER>
+__device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, int* s_p,
+                                              uint16_t* s_c, int* s_w16, int* s_state,
+                                              SelState& st) {
+  const int tid = threadIdx.x, lane = tid & 63, n = S.n;
+  const int win = min(KPER * 1024, top - st.consumed);
+  // coalesced loads: candidate i = t + 1024 k is the i-th from the top
+#pragma unroll
+  for (int k = 0; k < KPER; ++k) {
+    const int i = tid + 1024 * k;
+    int w = 0, c = 0;
+    if (i < win) {
+      const int si = top - 1 - (st.consumed + i);
+      const int l1 = S.s_l1[si], l2 = S.s_l2[si];
+      const int depth = (l1 + 1) + (n - l2);
+      const int u = n - depth;
+      w = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
+      c = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
+    }
+    s_p[sel_pad(i)] = w;
+    s_c[sel_pad(i)] = static_cast<uint16_t>(c);
+  }
+  __syncthreads();
+  // blocked inclusive scan: thread t owns candidates [KPER t, KPER t + KPER)
+  int loc[KPER], tsum = 0;
+#pragma unroll
+  for (int k = 0; k < KPER; ++k) {
+    tsum += s_p[sel_pad(KPER * tid + k)];
+    loc[k] = tsum;
+  }
+  int total;
+  const int off = block_exscan1024(tsum, s_w16, &total);
+#pragma unroll
+  for (int k = 0; k < KPER; ++k) s_p[sel_pad(KPER * tid + k)] = off + loc[k];
+  __syncthreads();
+  if (tid < 64) {
+    int end = win, stop = 0, base = 0, cur = st.carry, nbw = st.nb;
+    int i = wave_first_at_least(s_p, 0, win, S.pool_m1 - cur, lane);
+    while (i < win) {
+      ++nbw;
+      if (nbw == S.max_batches) {
+        stop = 1;
+        end = i;
+        break;
+      }
+      base = i > 0 ? s_p[sel_pad(i - 1)] : 0;
+      cur = 0;
+      i = wave_first_at_least(s_p, i + 1, win, base + S.pool_m1, lane);
+    }
+    if (lane == 0) {
+      const int lastp = s_p[sel_pad(win - 1)];
+      s_state[0] = end;
+      s_state[1] = stop;
+      s_state[2] = cur + (stop ? 0 : lastp - base);  // reservation carried on
+      s_state[3] = nbw;
+    }
+  }
+  __syncthreads();
+  const int end = s_state[0];
+  st.stop = s_state[1];
+  st.carry = s_state[2];
+  st.nb = s_state[3];
+  // parents taken in [0, end): candidates whose prefix steps up
+  int mine = 0, mc = 0;
+#pragma unroll
+  for (int k = 0; k < KPER; ++k) {
+    const int i = KPER * tid + k;
+    const int prev = i > 0 ? s_p[sel_pad(i - 1)] : 0;
+    const bool t = i < end && s_p[sel_pad(i)] > prev;
+    mine += t;
+    mc += t ? s_c[sel_pad(i)] : 0;
+  }
+  int ntaken, nchild;
+  int slot = st.np + block_exscan1024(mine, s_w16, &ntaken);
+  int coff = st.nc + block_exscan1024(mc, s_w16, &nchild);
+#pragma unroll
+  for (int k = 0; k < KPER; ++k) {
+    const int i = KPER * tid + k;
+    const int prev = i > 0 ? s_p[sel_pad(i - 1)] : 0;
+    if (i < end && s_p[sel_pad(i)] > prev) {
+      if (slot < S.PCAP) {
+        S.p_sel[slot] = top - 1 - (st.consumed + i);
+        S.p_off[slot] = coff;
+      }
+      ++slot;
+      coff += s_c[sel_pad(i)];
+    }
+  }
+  st.np += ntaken;
+  st.nc += nchild;
+  st.pruned += end - ntaken;
+  st.consumed += end;
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_select(SArgs S) {
-  __shared__ int s_p[kSelWin];  // prefix sums of reservations
-  __shared__ int s_c[kSelWin];  // children generated per candidate
+  __shared__ int s_p[kSelWin + kSelWin / 16];       // prefix sums of reservations
+  __shared__ uint16_t s_c[kSelWin + kSelWin / 16];  // children generated per candidate
   __shared__ int s_w16[40];
   __shared__ int s_state[4];
   Ctrl* C = S.ctrl;
-  const int tid = threadIdx.x, lane = tid & 63;
-  constexpr int kPer = kSelWin / 1024;
+  const int tid = threadIdx.x;
   if (C->status != kRunning) {
     if (tid == 0) {
       C->n_parents = 0;
@@ -187,7 +313,6 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     return;
   }
   const int top = C->stack_size;
-  const int n = S.n;
   if (top == 0) {
     if (tid == 0) {
       C->status = kDone;
@@ -198,94 +323,14 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   }
   const int inc = C->inc;
   // block-uniform carried state: reservation of the open sub-batch, closed
-  // sub-batches, parents taken, children, pruned candidates, consumed
-  int carry = 0, nb = 0, np = 0, nc = 0, pruned = 0, consumed = 0;
-  for (;;) {
-    const int win = min(kSelWin, top - consumed);
-    if (win <= 0) break;
-    // coalesced loads: candidate i = t + 1024 k is the i-th from the top
-    for (int k = 0; k < kPer; ++k) {
-      const int i = tid + 1024 * k;
-      int w = 0, c = 0;
-      if (i < win) {
-        const int si = top - 1 - (consumed + i);
-        const int l1 = S.s_l1[si], l2 = S.s_l2[si];
-        const int depth = (l1 + 1) + (n - l2);
-        const int u = n - depth;
-        w = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
-        c = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
-      }
-      s_p[i] = w;
-      s_c[i] = c;
-    }
-    __syncthreads();
-    // blocked inclusive scan: thread t owns candidates [kPer t, kPer t + kPer)
-    int loc[kPer], tsum = 0;
-    for (int k = 0; k < kPer; ++k) {
-      tsum += s_p[kPer * tid + k];
-      loc[k] = tsum;
-    }
-    int total;
-    const int off = block_exscan1024(tsum, s_w16, &total);
-    for (int k = 0; k < kPer; ++k) s_p[kPer * tid + k] = off + loc[k];
-    __syncthreads();
-    if (tid < 64) {
-      int end = win, stop = 0, base = 0, cur = carry, nbw = nb;
-      int i = wave_first_at_least(s_p, 0, win, S.pool_m1 - cur, lane);
-      while (i < win) {
-        ++nbw;
-        if (nbw == S.max_batches) {
-          stop = 1;
-          end = i;
-          break;
-        }
-        base = i > 0 ? s_p[i - 1] : 0;
-        cur = 0;
-        i = wave_first_at_least(s_p, i + 1, win, base + S.pool_m1, lane);
-      }
-      if (lane == 0) {
-        const int lastp = s_p[win - 1];
-        s_state[0] = end;
-        s_state[1] = stop;
-        s_state[2] = cur + (stop ? 0 : lastp - base);  // reservation carried on
-        s_state[3] = nbw;
-      }
-    }
-    __syncthreads();
-    const int end = s_state[0], stop = s_state[1];
-    carry = s_state[2];
-    nb = s_state[3];
-    // parents taken in [0, end): candidates whose prefix steps up
-    int mine = 0, mc = 0;
-    for (int k = 0; k < kPer; ++k) {
-      const int i = kPer * tid + k;
-      const int prev = i > 0 ? s_p[i - 1] : 0;
-      const bool t = i < end && s_p[i] > prev;
-      mine += t;
-      mc += t ? s_c[i] : 0;
-    }
-    int ntaken, nchild;
-    int slot = np + block_exscan1024(mine, s_w16, &ntaken);
-    int coff = nc + block_exscan1024(mc, s_w16, &nchild);
-    for (int k = 0; k < kPer; ++k) {
-      const int i = kPer * tid + k;
-      const int prev = i > 0 ? s_p[i - 1] : 0;
-      if (i < end && s_p[i] > prev) {
-        if (slot < S.PCAP) {
-          S.p_sel[slot] = top - 1 - (consumed + i);
-          S.p_off[slot] = coff;
-        }
-        ++slot;
-        coff += s_c[i];
-      }
-    }
-    np += ntaken;
-    nc += nchild;
-    pruned += end - ntaken;
-    consumed += end;
-    __syncthreads();
-    if (stop) break;
-  }
+  // sub-batches, parents taken, children, pruned candidates, consumed; a
+  // small first window (most rounds of small pools end in it), then large
+  // ones
+  SelState st{0, 0, 0, 0, 0, 0, 0};
+  select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_c, s_w16, s_state, st);
+  while (!st.stop && top - st.consumed > 0)
+    select_window<kSelWin / 1024>(S, top, inc, s_p, s_c, s_w16, s_state, st);
+  const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
   if (tid == 0) {
     if (np > S.PCAP || nc > S.CCAP) {
       C->status = kOverflow;
@@ -375,6 +420,19 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S) {
 }
 
 // ---- rank: one wave per parent -------------------------------------------
+// one DPP step of the leaf minimum: cost ascending, then job descending
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void leaf_step(int& lmin, int& lwin, int& lwjob) {
+  const int om = __builtin_amdgcn_update_dpp(INT_MAX, lmin, CTRL, RMASK, 0xf, false);
+  const int oj = __builtin_amdgcn_update_dpp(-1, lwjob, CTRL, RMASK, 0xf, false);
+  const int ow = __builtin_amdgcn_update_dpp(-1, lwin, CTRL, RMASK, 0xf, false);
+  if (om < lmin || (om == lmin && oj > lwjob)) {
+    lmin = om;
+    lwin = ow;
+    lwjob = oj;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_rank(SArgs S) {
   __shared__ int s_cost[4][kMaxJobs];
   __shared__ int s_job[4][kMaxJobs];
@@ -407,12 +465,10 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
           s1 += big ? 0 : b;
         }
       }
-      for (int d = 32; d >= 1; d >>= 1) {
-        c0 += __shfl_xor(c0, d, 64);
-        c1 += __shfl_xor(c1, d, 64);
-        s0 += __shfl_xor(s0, d, 64);
-        s1 += __shfl_xor(s1, d, 64);
-      }
+      c0 = wave_reduce(c0, 0, OpAdd());
+      c1 = wave_reduce(c1, 0, OpAdd());
+      s0 = wave_reduce(s0, 0, OpAdd());
+      s1 = wave_reduce(s1, 0, OpAdd());
       side = (c0 > c1) ? 0 : (c1 > c0) ? 1 : (s0 > s1) ? 0 : 1;
       nch = simple ? (side == 0 ? 2 : 0) : u;
       const int first = simple ? 0 : side * u;
@@ -455,18 +511,20 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
         }
       }
       if (simple) {
-        for (int d = 32; d >= 1; d >>= 1) {
-          const int om = __shfl_xor(lmin, d, 64), ow = __shfl_xor(lwin, d, 64),
-                    oj = __shfl_xor(lwjob, d, 64);
-          if (om < lmin || (om == lmin && oj > lwjob)) {
-            lmin = om;
-            lwin = ow;
-            lwjob = oj;
-          }
-          lmax = max(lmax, __shfl_xor(lmax, d, 64));
-        }
+        // scan of (cost ascending, job descending) with the child index as
+        // payload, same DPP steps as wave_incl_scan
+        leaf_step<0x111, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x112, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x114, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x118, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x142, 0xa>(lmin, lwin, lwjob);
+        leaf_step<0x143, 0xc>(lmin, lwin, lwjob);
+        lmin = __builtin_amdgcn_readlane(lmin, 63);
+        lwin = __builtin_amdgcn_readlane(lwin, 63);
+        lwjob = __builtin_amdgcn_readlane(lwjob, 63);
+        lmax = wave_reduce(lmax, INT_MIN, OpMax());
       }
-      for (int d = 32; d >= 1; d >>= 1) below += __shfl_xor(below, d, 64);
+      below = wave_reduce(below, 0, OpAdd());
       if (lane == 0) {
         S.p_kept0[p] = below;
         S.p_side[p] = side;
@@ -484,16 +542,34 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
 }
 
 // ---- settle: incumbent per parent, kept children, survivor offsets --------
-// One block, parents in chunks of 4096 (thread t owns four consecutive
-// parents).  The incumbent a parent's children are pruned against is the
-// running minimum of the leaves of the parents popped before it
-// (solutions::insert strictly-better rule, solutions.cpp:45-65; prune loop
-// gpu.cu:317-394).  Its kept children are the ranks whose bound is below
-// that incumbent: counted by the rank kernel against the round's starting
-// incumbent, searched again only for parents after an improvement.  Prefix
-// sums of the kept counts place the survivors on the stack.
+// One block, parents in chunks of 16384 (thread t owns 16 consecutive
+// parents, read and written with 16-byte vector accesses; a round of a
+// 65536-child pool fits one chunk).  The incumbent a parent's children are
+// pruned against is the running minimum of the leaves of the parents popped
+// before it (solutions::insert strictly-better rule, solutions.cpp:45-65;
+// prune loop gpu.cu:317-394).  Its kept children are the ranks whose bound is
+// below that incumbent: counted by the rank kernel against the round's
+// starting incumbent, searched again only for parents after an improvement.
+// Prefix sums of the kept counts place the survivors on the stack.
+__device__ __forceinline__ void ld16(const int* a, int (&v)[kSettleK]) {
+  const int4* q = reinterpret_cast<const int4*>(a);
+#pragma unroll
+  for (int i = 0; i < kSettleK / 4; ++i) {
+    const int4 x = q[i];
+    v[4 * i] = x.x;
+    v[4 * i + 1] = x.y;
+    v[4 * i + 2] = x.z;
+    v[4 * i + 3] = x.w;
+  }
+}
+__device__ __forceinline__ void st16(int* a, const int (&v)[kSettleK]) {
+  int4* q = reinterpret_cast<int4*>(a);
+#pragma unroll
+  for (int i = 0; i < kSettleK / 4; ++i) q[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+}
+
 __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
-  constexpr int K = 4;
+  constexpr int K = kSettleK;
   __shared__ int s_w16[40];
   __shared__ int s_last;
   Ctrl* C = S.ctrl;
@@ -506,12 +582,24 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   int improved = 0, last = -1;
   long long pruned = 0;
   for (int c0 = 0; c0 < np; c0 += 1024 * K) {
-    int lmin[K], kept[K];
+    const int base = c0 + K * tid;  // arrays hold PCAP + 1 + K entries
+    const bool any = base < np;
+    int lmin[K], kept[K], k0[K], nc[K];
+    uint32_t inner = 0;  // bit k: parent base + k has more than two leaves' depth left
+    if (any) {
+      int a1[K], a2[K];
+      ld16(S.p_l1 + base, a1);
+      ld16(S.p_l2 + base, a2);
+#pragma unroll
+      for (int k = 0; k < K; ++k) inner |= static_cast<uint32_t>(((a1[k] + 1) + (n - a2[k])) != n - 2) << k;
+      ld16(S.p_leafmin + base, lmin);
+      ld16(S.p_kept0 + base, k0);
+      ld16(S.p_nch + base, nc);
+    }
     int tmin = INT_MAX;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int p = c0 + K * tid + k;
-      lmin[k] = p < np ? S.p_leafmin[p] : INT_MAX;
+      if (!(base + k < np)) lmin[k] = INT_MAX;
       tmin = min(tmin, lmin[k]);
     }
     int cmin;
@@ -519,7 +607,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
     int tkept = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int p = c0 + K * tid + k;
+      const int p = base + k;
       kept[k] = 0;
       if (p < np) {
         if (lmin[k] < ib) {
@@ -529,15 +617,13 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
           improved += (lmax < ib) + (lmin[k] < min(ib, lmax));
           last = p;
         }
-        const int l1 = S.p_l1[p], l2 = S.p_l2[p];
-        if (((l1 + 1) + (n - l2)) != n - 2) {
-          const int nch = S.p_nch[p];
+        if ((inner >> k) & 1u) {
           int kp;
           if (ib == inc0) {
-            kp = S.p_kept0[p];
+            kp = k0[k];
           } else {
             const int off = S.p_off[p];
-            int lo = 0, hi = nch;  // first rank whose bound is >= ib
+            int lo = 0, hi = nc[k];  // first rank whose bound is >= ib
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
               if (S.c_scost[off + mid] < ib) lo = mid + 1;
@@ -546,7 +632,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
             kp = lo;
           }
           kept[k] = kp;
-          pruned += nch - kp;
+          pruned += nc[k] - kp;
         }
         tkept += kept[k];
       }
@@ -554,14 +640,15 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
     }
     int csum;
     int koff = run_kept + block_exscan1024(tkept, s_w16, &csum);
+    if (any) {
+      st16(S.p_kept + base, kept);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int p = c0 + K * tid + k;
-      if (p < np) {
-        S.p_kept[p] = kept[k];
-        S.p_koff[p] = koff;
-        koff += kept[k];
+      for (int k = 0; k < K; ++k) {
+        const int x = kept[k];
+        kept[k] = koff;  // becomes the survivor offset
+        koff += x;
       }
+      st16(S.p_koff + base, kept);
     }
     run_min = min(run_min, cmin);
     run_kept += csum;
@@ -583,14 +670,14 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   const int bl = s_last;
   if (tid == 0) {
     if (bl >= 0) C->inc = run_min;
-    const int base = C->stack_size;
-    C->push_base = base;
+    const int sbase = C->stack_size;
+    C->push_base = sbase;
     C->push_total = run_kept;
-    if (static_cast<long long>(base) + run_kept > S.CAP) {
+    if (static_cast<long long>(sbase) + run_kept > S.CAP) {
       C->status = kOverflow;
     } else {
-      C->stack_size = base + run_kept;
-      if (base + run_kept > C->stats[kStMaxPool]) C->stats[kStMaxPool] = base + run_kept;
+      C->stack_size = sbase + run_kept;
+      if (sbase + run_kept > C->stats[kStMaxPool]) C->stats[kStMaxPool] = sbase + run_kept;
     }
   }
   // the leaf that set the incumbent: parent permutation with its swap
@@ -817,8 +904,10 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_sl2 = take(4 * size_t(cap_nodes));
   const size_t o_scost = take(4 * size_t(cap_nodes));
   const size_t o_pperm = take(size_t(n) * PCAP);
+  // parent arrays: PCAP + 1 entries plus room for the settle kernel's
+  // 16-wide vector accesses past the last parent
   size_t o_pint[13];
-  for (int i = 0; i < 13; ++i) o_pint[i] = take(4 * size_t(PCAP + 1));
+  for (int i = 0; i < 13; ++i) o_pint[i] = take(4 * size_t(PCAP + 1 + kSettleK));
   size_t o_cint[4];
   for (int i = 0; i < 4; ++i) o_cint[i] = take(4 * size_t(CCAP));
   hipError_t e = S->mem.reserve(off);
