@@ -92,6 +92,31 @@ __device__ __forceinline__ void step4(const StepGroup<NW>& g, const uint32_t (&u
 // MODE 2: MODE 1 plus LB1 and the selector features; MODE 3: MODE 0 over the
 // pairs of each tile's subset (tile_sel) only.  Modes 0 and 3 write their
 // result to the source's slot of the node.
+// Processing times staged in LDS job-major, [n][MM] with machines m..MM-1
+// zero: the head/tail recurrences read a job's machines as MM/4 16-byte
+// loads from one address instead of MM loads from MM rows.
+template <int MM>
+__device__ __forceinline__ void stage_times(const DevTables& T, int32_t* sp) {
+  static_assert(MM % 4 == 0, "machine rows are read four at a time");
+  const int n = T.n, m = T.m;
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) {
+    const int j = i / MM, k = i - j * MM;
+    sp[i] = k < m ? T.p[k * n + j] : 0;
+  }
+}
+template <int MM>
+__device__ __forceinline__ void job_row(const int32_t* sp, int j, int (&pk)[MM]) {
+  const int4* rp = reinterpret_cast<const int4*>(sp + j * MM);
+#pragma unroll
+  for (int k4 = 0; k4 < MM / 4; ++k4) {
+    const int4 v = rp[k4];
+    pk[4 * k4] = v.x;
+    pk[4 * k4 + 1] = v.y;
+    pk[4 * k4 + 2] = v.z;
+    pk[4 * k4 + 3] = v.w;
+  }
+}
+
 template <int MM, int NW, int MODE, int BLOCK, class Src>
 __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, BoundArgs A) {
   constexpr bool LEAN = MODE == 0 || MODE == 3, EXTRA = MODE == 2;
@@ -101,13 +126,13 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   // over all MM machines without per-machine guards, and the padded heads
   // and tails never exceed the real ones
   const int mn4 = (MM * n + 3) & ~3;
-  int32_t* sp = lds;                                 // [MM][n]
+  int32_t* sp = lds;                                 // [n][MM]
   int4* part = reinterpret_cast<int4*>(lds + mn4);   // [blockDim] partials
   // [n][64 * tiles per block] bytes: the permutation columns of the block's
   // tiles, staged once per tile so the head/tail recurrences read LDS
   // instead of waiting on one global load per position
   uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);
-  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
+  stage_times<MM>(T, sp);
   __syncthreads();
 
   const int G = A.groups;
@@ -154,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     const int bstride = 64 * tpb;
     if (grp == 0 && valid) {
       for (int i0 = 0; i0 < n; i0 += 16) {
-        uint32_t v[16];
+        uint16_t v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u)
           if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
@@ -178,20 +203,29 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
 #pragma unroll
     for (int w = 0; w < NW; ++w) sm[w] = 0;
 
+    // the next job index is read while the current row is processed
+    int jn = d1 > 0 ? perm_at(mine, bstride, sw0, sw1, 0) : 0;
     for (int i = 0; i < d1; ++i) {
-      const int j = perm_at(mine, bstride, sw0, sw1, i);
+      const int j = jn;
+      if (i + 1 < d1) jn = perm_at(mine, bstride, sw0, sw1, i + 1);
       set_job_bit<NW>(sm, j);
-      r[0] += sp[j];
+      int pk[MM];
+      job_row<MM>(sp, j, pk);
+      r[0] += pk[0];
 #pragma unroll
       for (int k = 1; k < MM; ++k)
-        r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
+        r[k] = max(r[k], r[k - 1]) + pk[k];
     }
+    jn = d2 > 0 ? perm_at(mine, bstride, sw0, sw1, n - 1) : 0;
     for (int i = n - 1; i >= l2; --i) {
-      const int j = perm_at(mine, bstride, sw0, sw1, i);
+      const int j = jn;
+      if (i - 1 >= l2) jn = perm_at(mine, bstride, sw0, sw1, i - 1);
       set_job_bit<NW>(sm, j);
+      int pk[MM];
+      job_row<MM>(sp, j, pk);
 #pragma unroll
       for (int k = MM - 1; k >= 0; --k)
-        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
+        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
     }
     const bool leaf = (l2 - l1 == 1);
     int leafv = 0;
@@ -209,9 +243,11 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
       const int rem_hi = valid ? l2 : d1;  // padding lanes read no permutation
       for (int i = d1; i < rem_hi; ++i) {
         const int j = perm_at(mine, bstride, sw0, sw1, i);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
 #pragma unroll
         for (int k = 0; k < MM; ++k)
-          rem[k] += sp[k * n + j];
+          rem[k] += pk[k];
       }
 #pragma unroll
       for (int k = 0; k < MM; ++k)
@@ -253,9 +289,24 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     int npairs = 0;  // pairs scanned by this wave
     if constexpr (LEAN) {
       npairs = s_hi - s_lo;
-      for (int s = s_lo; s < s_hi; ++s) {
-        if constexpr (MODE == 3)
-          if (!((ld_const_u32(tsel, s >> 5) >> (s & 31)) & 1u)) continue;
+      // MODE 3 visits the set bits of the tile's mask words in order (the
+      // mask words are wave-uniform scalars); MODE 0 every pair in range
+      uint32_t word = 0;
+      int wi = -1;
+      for (int s = s_lo;;) {
+        if constexpr (MODE == 3) {
+          while (word == 0) {
+            if (++wi >= A.nsw) break;
+            word = ld_const_u32(tsel, wi);
+          }
+          if (word == 0) break;
+          s = 32 * wi + __builtin_ctz(word);
+          word &= word - 1;
+          if (s < s_lo) continue;  // another group's share of the pairs
+          if (s >= s_hi) break;
+        } else {
+          if (s >= s_hi) break;
+        }
         const int2 ab = ld_const2(T.pairs, s);
         int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
         int t1 = r[ab.y];
@@ -271,6 +322,7 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
         const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + q0b;
         lb = max(lb, v);
         if (use_best && fx < 0 && lb > A.best) fx = lb;
+        if constexpr (MODE != 3) ++s;
       }
     } else {
     const int npass = (tsel != nullptr && A.margin >= 0) ? 2 : 1;
@@ -413,10 +465,10 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
   const int mn4 = (MM * n + 3) & ~3;
-  int32_t* sp = lds;                                           // [MM][n]
+  int32_t* sp = lds;                                           // [n][MM]
   uint32_t* part = reinterpret_cast<uint32_t*>(lds + mn4);     // [blockDim]
   uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);  // [n][128 tpb]
-  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) sp[i] = i < m * n ? T.p[i] : 0;
+  stage_times<MM>(T, sp);
   __syncthreads();
   const int G = A.groups;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -453,7 +505,7 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
       }
       if (grp == 0 && valid[h]) {
         for (int i0 = 0; i0 < n; i0 += 16) {
-          uint32_t v[16];
+          uint16_t v[16];
 #pragma unroll
           for (int u = 0; u < 16; ++u)
             if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
@@ -479,18 +531,26 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
       for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) sm[w] = 0;
+      int jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
       for (int i = 0; i < d1; ++i) {
-        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        const int j = jn;
+        if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
         set_job_bit<NW>(sm, j);
-        r[0] += sp[j];
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        r[0] += pk[0];
 #pragma unroll
-        for (int k = 1; k < MM; ++k) r[k] = max(r[k], r[k - 1]) + sp[k * n + j];
+        for (int k = 1; k < MM; ++k) r[k] = max(r[k], r[k - 1]) + pk[k];
       }
+      jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
       for (int i = n - 1; i >= l2[h]; --i) {
-        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        const int j = jn;
+        if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
         set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
 #pragma unroll
-        for (int k = MM - 1; k >= 0; --k) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + sp[k * n + j];
+        for (int k = MM - 1; k >= 0; --k) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
       }
       int lv = 0;
 #pragma unroll
@@ -606,10 +666,7 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
   const int n = T.n, m = T.m;
   int32_t* sp = lds;  // [n][MM] job-major, machines m..MM-1 zero
   uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * n);  // [n][256]
-  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) {
-    const int j = i / MM, k = i - j * MM;
-    sp[i] = k < m ? T.p[k * n + j] : 0;
-  }
+  stage_times<MM>(T, sp);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // feature f = g m + k sums rows ra + rb of hv (row 3 MM is zero): one entry
   // ra | rb << 16 per feature, zero rows for the padding features
@@ -655,23 +712,12 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
     int r[MM], q[MM], sc_[MM];
 #pragma unroll
     for (int k = 0; k < MM; ++k) r[k] = q[k] = sc_[k] = 0;
-    auto row = [&](int j, int (&pk)[MM]) {
-      const int4* rp = reinterpret_cast<const int4*>(sp + j * MM);
-#pragma unroll
-      for (int k4 = 0; k4 < MM / 4; ++k4) {
-        const int4 v = rp[k4];
-        pk[4 * k4] = v.x;
-        pk[4 * k4 + 1] = v.y;
-        pk[4 * k4 + 2] = v.z;
-        pk[4 * k4 + 3] = v.w;
-      }
-    };
     int jn = d1 > 0 ? perm_at(mine, 256, sw0, sw1, 0) : 0;
     for (int i = 0; i < d1; ++i) {
       const int j = jn;
       if (i + 1 < d1) jn = perm_at(mine, 256, sw0, sw1, i + 1);
       int pk[MM];
-      row(j, pk);
+      job_row<MM>(sp, j, pk);
       r[0] += pk[0];
       sc_[0] += pk[0];
 #pragma unroll
@@ -685,7 +731,7 @@ __global__ __launch_bounds__(256) void k_features(DevTables T, Src src, FeatArgs
       const int j = jn;
       if (i - 1 >= l2) jn = perm_at(mine, 256, sw0, sw1, i - 1);
       int pk[MM];
-      row(j, pk);
+      job_row<MM>(sp, j, pk);
 #pragma unroll
       for (int k = MM - 1; k >= 0; --k) {
         q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
