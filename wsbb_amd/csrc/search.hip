@@ -161,6 +161,39 @@ __device__ __forceinline__ int block_exscan1024(int v, int* s_w16, int* total) {
   return block_exscan1024(v, 0, OpAdd(), s_w16, total);
 }
 
+// two exclusive sums over a 1024-thread block sharing one set of barriers
+// (s_w needs 80 ints)
+__device__ __forceinline__ void block_exscan1024_2(int a, int b, int* s_w, int* ea, int* eb,
+                                                   int* ta, int* tb) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ia = wave_incl_scan(a, 0, OpAdd()), ib = wave_incl_scan(b, 0, OpAdd());
+  const int xa = wave_shr1(ia, 0), xb = wave_shr1(ib, 0);
+  if (lane == 63) {
+    s_w[wave] = ia;
+    s_w[40 + wave] = ib;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const int va = lane < 16 ? s_w[lane] : 0, vb = lane < 16 ? s_w[40 + lane] : 0;
+    const int sa = wave_incl_scan(va, 0, OpAdd()), sb = wave_incl_scan(vb, 0, OpAdd());
+    const int pa = wave_shr1(sa, 0), pb = wave_shr1(sb, 0);
+    if (lane < 16) {
+      s_w[16 + lane] = pa;
+      s_w[56 + lane] = pb;
+    }
+    if (lane == 15) {
+      s_w[32] = sa;
+      s_w[72] = sb;
+    }
+  }
+  __syncthreads();
+  *ea = s_w[16 + wave] + xa;
+  *eb = s_w[56 + wave] + xb;
+  *ta = s_w[32];
+  *tb = s_w[72];
+  __syncthreads();
+}
+
 // first index i in [lo, hi) with a[i] >= thr for nondecreasing a (hi if
 // none), by the whole wave: each round probes the last element of 64 equal
 // buckets and keeps the first bucket that reaches thr, until at most 64
@@ -205,11 +238,12 @@ struct SelState {
 
 template <int KPER>
 __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, int* s_p,
-                                              uint16_t* s_c, int* s_w16, int* s_state,
-                                              SelState& st) {
+                                              int* s_w16, int* s_state, SelState& st) {
   const int tid = threadIdx.x, lane = tid & 63, n = S.n;
   const int win = min(KPER * 1024, top - st.consumed);
-  // coalesced loads: candidate i = t + 1024 k is the i-th from the top
+  // coalesced loads: candidate i = t + 1024 k is the i-th from the top; its
+  // reservation w (push_back1) and children c share one LDS word (both
+  // at most 2n)
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
     const int i = tid + 1024 * k;
@@ -222,15 +256,17 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
       w = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
       c = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
     }
-    s_p[sel_pad(i)] = w;
-    s_c[sel_pad(i)] = static_cast<uint16_t>(c);
+    s_p[sel_pad(i)] = w | (c << 16);
   }
   __syncthreads();
   // blocked inclusive scan: thread t owns candidates [KPER t, KPER t + KPER)
-  int loc[KPER], tsum = 0;
+  // and keeps their words in registers; the prefix sums go back to LDS for
+  // the boundary search
+  int word[KPER], loc[KPER], tsum = 0;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
-    tsum += s_p[sel_pad(KPER * tid + k)];
+    word[k] = s_p[sel_pad(KPER * tid + k)];
+    tsum += word[k] & 0xffff;
     loc[k] = tsum;
   }
   int total;
@@ -265,43 +301,39 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
   st.stop = s_state[1];
   st.carry = s_state[2];
   st.nb = s_state[3];
-  // parents taken in [0, end): candidates whose prefix steps up
+  // parents taken in [0, end): candidates whose prefix steps up (w > 0)
   int mine = 0, mc = 0;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
-    const int i = KPER * tid + k;
-    const int prev = i > 0 ? s_p[sel_pad(i - 1)] : 0;
-    const bool t = i < end && s_p[sel_pad(i)] > prev;
+    const bool t = KPER * tid + k < end && (word[k] & 0xffff) != 0;
     mine += t;
-    mc += t ? s_c[sel_pad(i)] : 0;
+    mc += t ? (word[k] >> 16) : 0;
   }
-  int ntaken, nchild;
-  int slot = st.np + block_exscan1024(mine, s_w16, &ntaken);
-  int coff = st.nc + block_exscan1024(mc, s_w16, &nchild);
+  int slot, coff, ntaken, nchild;
+  block_exscan1024_2(mine, mc, s_w16, &slot, &coff, &ntaken, &nchild);
+  slot += st.np;
+  coff += st.nc;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
     const int i = KPER * tid + k;
-    const int prev = i > 0 ? s_p[sel_pad(i - 1)] : 0;
-    if (i < end && s_p[sel_pad(i)] > prev) {
+    if (i < end && (word[k] & 0xffff) != 0) {
       if (slot < S.PCAP) {
         S.p_sel[slot] = top - 1 - (st.consumed + i);
         S.p_off[slot] = coff;
       }
       ++slot;
-      coff += s_c[sel_pad(i)];
+      coff += word[k] >> 16;
     }
   }
   st.np += ntaken;
   st.nc += nchild;
   st.pruned += end - ntaken;
   st.consumed += end;
-  __syncthreads();
 }
 
 __global__ __launch_bounds__(1024) void k_select(SArgs S) {
-  __shared__ int s_p[kSelWin + kSelWin / 16];       // prefix sums of reservations
-  __shared__ uint16_t s_c[kSelWin + kSelWin / 16];  // children generated per candidate
-  __shared__ int s_w16[40];
+  __shared__ int s_p[kSelWin + kSelWin / 16];  // candidate words, then prefix sums
+  __shared__ int s_w16[80];
   __shared__ int s_state[4];
   Ctrl* C = S.ctrl;
   const int tid = threadIdx.x;
@@ -327,9 +359,9 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   // small first window (most rounds of small pools end in it), then large
   // ones
   SelState st{0, 0, 0, 0, 0, 0, 0};
-  select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_c, s_w16, s_state, st);
+  select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   while (!st.stop && top - st.consumed > 0)
-    select_window<kSelWin / 1024>(S, top, inc, s_p, s_c, s_w16, s_state, st);
+    select_window<kSelWin / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
   if (tid == 0) {
     if (np > S.PCAP || nc > S.CCAP) {
