@@ -286,8 +286,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("all", "solve", "batched", "copy"), default="all",
-                    help="solve = ta011 time-to-optimum only; batched = 50x20 launches only")
+    ap.add_argument("--workload", choices=("all", "solve", "batched", "copy", "learned"),
+                    default="all",
+                    help="solve = ta011 time-to-optimum only; batched = 50x20 launches only; "
+                         "learned = the ta021 plain-against-learned comparison only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
@@ -311,6 +313,9 @@ def main():
         eng.free(dst)
         print(json.dumps({"workload": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
                           "ms": ms, "GBps": 2 * nbytes / ms / 1e6}), flush=True)
+        return
+    if args.workload == "learned":
+        print(json.dumps({"learned_20x20": bench_learned(wsbb_amd)}), flush=True)
         return
     if args.workload == "batched":
         bt = bench_batched(wsbb_amd, reps=args.steps, autotune=not args.no_autotune)
