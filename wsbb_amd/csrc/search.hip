@@ -356,10 +356,13 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   const int inc = C->inc;
   // block-uniform carried state: reservation of the open sub-batch, closed
   // sub-batches, parents taken, children, pruned candidates, consumed; a
-  // small first window (most rounds of small pools end in it), then large
-  // ones
+  // small first window for small pools (most of their rounds end in it),
+  // then large ones
   SelState st{0, 0, 0, 0, 0, 0, 0};
-  select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_w16, s_state, st);
+  // a round reserves about pool_size x sub-batches / (2u) parents: large
+  // pools start with the large window
+  if (static_cast<long long>(S.pool_m1) * S.max_batches <= 16LL * kSelWin0)
+    select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   while (!st.stop && top - st.consumed > 0)
     select_window<kSelWin / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;

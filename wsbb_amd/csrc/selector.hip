@@ -27,6 +27,7 @@
 #include "../../include/wsbb_amd.h"
 #include "context.h"
 #include "selector.h"
+#include "tiles.h"
 
 namespace wsbb {
 
@@ -74,44 +75,6 @@ struct MlpArgs {
   uint16_t* probs;          // [N][ldp] BF16 softmax probabilities (OUT == 2)
   int ldp;                  // multiple of 4, >= 32 PB
 };
-
-// Top-k of a wave's non-negative candidates without sorting.  Candidate
-// (i, lane) has number 64 i + lane; the bit pattern of a non-negative float
-// orders like its value, so the k-th largest value T is built bit by bit
-// from ballot counts (31 rounds of compares and scalar popcounts, no lane
-// exchanges), then sel[i] holds the candidates above T and, of those equal
-// to T, the lowest-numbered ones until k are chosen.
-template <int NI>
-__device__ __forceinline__ void wave_topk(const float (&v)[NI], const bool (&ok)[NI], int k,
-                                          unsigned long long (&sel)[NI]) {
-  uint32_t u[NI];
-#pragma unroll
-  for (int i = 0; i < NI; ++i) u[i] = v[i] > 0.f ? __float_as_uint(v[i]) : 0u;
-  uint32_t T = 0;
-  for (int bit = 30; bit >= 0; --bit) {
-    const uint32_t cand = T | (1u << bit);
-    int cnt = 0;
-#pragma unroll
-    for (int i = 0; i < NI; ++i) cnt += __popcll(__ballot(ok[i] && u[i] >= cand));
-    if (cnt >= k) T = cand;
-  }
-  int rem = k;
-  unsigned long long eq[NI];
-#pragma unroll
-  for (int i = 0; i < NI; ++i) {
-    sel[i] = __ballot(ok[i] && u[i] > T);
-    rem -= __popcll(sel[i]);
-    eq[i] = __ballot(ok[i] && u[i] == T);
-  }
-#pragma unroll
-  for (int i = 0; i < NI; ++i)
-    while (rem > 0 && eq[i]) {
-      const unsigned long long low = eq[i] & (~eq[i] + 1ull);
-      sel[i] |= low;
-      eq[i] ^= low;
-      --rem;
-    }
-}
 
 __device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
   bf16x4 r;
@@ -323,12 +286,10 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   }
 }
 
-// Pair subsets of child tiles from parent-level predictions.  Children of a
-// parent are consecutive, so the parents of a 64-child tile are the range
-// from the first child's parent to the last child's; each adds its
-// probabilities weighted by its number of children in the tile.  One wave per
-// tile, lane l sums pairs 64 i + l; top-k as in the MLP kernel (ties: lower
-// pair index).
+// Pair subsets of child tiles from parent-level predictions
+// (parent_tile_mask), one wave per tile.  The learned search computes the
+// same masks inside its subset bound kernel; this launch serves the
+// per-parent inference entry point.
 struct ParentTileArgs {
   const uint16_t* probs;  // [parents][ldp] BF16
   int ldp, P, nsw, topk;
@@ -347,39 +308,8 @@ __global__ __launch_bounds__(256) void k_parent_tiles(ParentTileArgs A) {
     const int tile = __builtin_amdgcn_readfirstlane(t0 + wave);
     if (tile >= ntiles) continue;
     const int c0 = tile * 64, c1 = min(c0 + 64, nc);
-    // descriptors and offsets were written by earlier launches: scalar loads
-    const int pa = ld_const(A.desc, c0) >> 9;
-    const int pb = ld_const(A.desc, c1 - 1) >> 9;
-    float agg[NI];
-#pragma unroll
-    for (int i = 0; i < NI; ++i) agg[i] = 0.f;
-    // eight parents per batch: every row load of a batch is in flight at once
-    for (int p0 = pa; p0 <= pb; p0 += 8) {
-      float w[8], x[8][NI];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int p = p0 + j;
-        const bool in = p <= pb;
-        const int lo = in ? max(ld_const(A.p_off, p), c0) : 0;
-        const int hi = in ? min(ld_const(A.p_off, p + 1), c1) : 0;
-        w[j] = static_cast<float>(max(hi - lo, 0));
-        const uint16_t* row = A.probs + static_cast<size_t>(in ? p : pa) * A.ldp;
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int s = 64 * i + lane;
-          x[j][i] = s < A.ldp ? bf2f(row[s]) : 0.f;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-#pragma unroll
-        for (int i = 0; i < NI; ++i) agg[i] += w[j] * x[j][i];
-    }
-    bool ok[NI];
-#pragma unroll
-    for (int i = 0; i < NI; ++i) ok[i] = 64 * i + lane < A.P;
     unsigned long long sel[NI];
-    wave_topk<NI>(agg, ok, min(A.topk, A.P), sel);
+    parent_tile_mask<NI>(A.probs, A.ldp, A.P, A.topk, A.p_off, A.desc, c0, c1, lane, sel);
     // candidates 64 i .. 64 i + 63 = mask words 2i and 2i + 1
     if (lane == 0) {
       uint32_t* ts = A.tile_sel + static_cast<size_t>(tile) * A.nsw;
