@@ -55,14 +55,17 @@ def node_work(n, m, l1, l2):
     return P * (n + 4 * u + 4) + 2 * m * d1 + m * (m + 1) * d2
 
 
+TIME_EVERY = 8  # the bound launch of every 8th round carries HIP events
+
+
 def bench_solve(wsbb_amd, steps, warmup):
     p = wsbb_amd.taillard(11)
     eng = wsbb_amd.Engine(p)
-    eng.search_profile(1)
+    eng.search_profile(TIME_EVERY)
     for _ in range(warmup):
         eng.solve(pool_size=4096, max_batches=10)
     eng.synchronize()
-    tot = dict(bounded=0, work=0, bound_ms=0.0, launches=0)
+    tot = dict(bounded=0, work=0, bound_ms=0.0, launches=0, rounds=0)
     best = None
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -72,6 +75,7 @@ def bench_solve(wsbb_amd, steps, warmup):
         tot["work"] += res["stats"]["work_ops"]
         tot["bound_ms"] += ms
         tot["launches"] += launches
+        tot["rounds"] += res["stats"]["rounds"]
         best = res["best"]
         assert res["proven"]
     eng.synchronize()
@@ -330,7 +334,9 @@ def main():
         cpu_solve, cpu_b = cpu_baselines(bt)
 
     value = sv["bounded"] / sv["elapsed"]
-    bound_s = sv["bound_ms"] / 1e3
+    # average bound-launch time from the timed rounds, times every round
+    avg_ms = sv["bound_ms"] / max(sv["launches"], 1)
+    bound_s = avg_ms * sv["rounds"] / 1e3
     ach = sv["work"] / bound_s / 1e12 if bound_s > 0 else None
     line = {
         "metric": "LB2 bounded nodes/s + time-to-proven-optimum, 1 MI355X vs host CPU",
@@ -358,8 +364,11 @@ def main():
                      "traffic": measured_traffic("solve"),
                      "algorithmic_bytes": sv["stats"]["bounded"] / max(sv["stats"]["rounds"], 1)
                      * (20 + 12),
-                     "avg_launch_us": sv["bound_ms"] * 1e3 / max(sv["launches"], 1),
-                     "launches": sv["launches"],
+                     "avg_launch_us": avg_ms * 1e3,
+                     "launches": sv["rounds"],
+                     "timed_launches": sv["launches"],
+                     "timing": f"HIP events on the bound launch of every {TIME_EVERY}th round "
+                               "inside the timed solves (the launch records them itself)",
                      "share_of_solve": bound_s / sv["elapsed"]},
     }
     if cpu_solve:

@@ -168,9 +168,12 @@ int wsbb_solve(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
                int32_t ub0, int64_t budget, int32_t *best,
                int32_t *best_perm, int64_t *stats);
 /* Optional HIP-event timing of the bound-kernel launches of the next
- * wsbb_solve calls (enable = 1/0; -1 only reads).  Reports the summed kernel
- * time and the number of launches of the last solve.  No counterpart in the
- * reference (its statistics time whole GPU phases, explorer.cu:219-249). */
+ * wsbb_solve calls: enable = k > 0 times the bound launch of every k-th round
+ * (events taken by the launch itself), 0 turns timing off, -1 only reads.
+ * Reports the summed kernel time and the number of timed launches of the
+ * last solve (rounds after the search ended are not counted).  No
+ * counterpart in the reference (its statistics time whole GPU phases,
+ * explorer.cu:219-249). */
 int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
                         int64_t *bound_launches);
 /* Device node-pool capacity in nodes (default 2^24) for later solves. */

@@ -182,11 +182,21 @@ def test_parent_selector_logging_and_solves():
         losses = eng.selector_train(epochs=4, batch=512, lr=2e-3, seed=2)
         assert losses[-1] < losses[0]
         fb = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+        # the host keeps a batch of rounds queued behind the one it checks;
+        # rounds after the end must leave every counter alone, whatever the
+        # batch size (the fallback count included)
+        eng.search_options(rounds_per_sync=1)
+        fb1 = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+        eng.search_options(rounds_per_sync=32)
+        fb32 = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
+        eng.search_options(rounds_per_sync=8)
         eng.selector_config(topk=8, margin=-1)
         nofb = eng.solve(kind=wsbb_amd.LEARNED, pool_size=4096, max_batches=10)
     for r in (fb, nofb):
         assert r["proven"] and r["best"] == plain["best"]
         assert r["stats"]["subset_bounded"] == r["stats"]["bounded"]
+    assert fb1["stats"] == fb["stats"] == fb32["stats"]
+    assert fb["stats"]["fallbacks"] > 0
     assert nofb["stats"]["fallbacks"] == 0
     print(f"parents: nodes plain {plain['stats']['bounded']} learned+fallback "
           f"{fb['stats']['bounded']} ({fb['stats']['fallbacks']} fallbacks) learned "

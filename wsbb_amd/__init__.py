@@ -291,6 +291,9 @@ class Engine:
 
     # ---- search ----
     def search_profile(self, enable=-1):
+        """enable = k > 0: time the bound launch of every k-th round of later
+        solves (0 off, -1 keep); returns (summed ms, timed launches) of the
+        last solve."""
         ms = ctypes.c_double()
         cnt = ctypes.c_int64()
         _check(lib.wsbb_search_profile(self.h, enable, ctypes.byref(ms), ctypes.byref(cnt)),

@@ -21,6 +21,8 @@
 // (heuristiqueCmax 64-77) and the tail combine (cmaxFin 92-100, 147-148).
 #include "bound.h"
 
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 namespace wsbb {
@@ -623,7 +625,8 @@ hipError_t launch_pk(const DevTables& T, const Src& src, const BoundArgs& A, int
   if (grid < 1) grid = 1;
   const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + block) +
                        static_cast<size_t>(T.n) * 128 * tpb;
-  hipLaunchKernelGGL((bound_kernel_pk<MM, NW, Src>), dim3(grid), dim3(block), shmem, stream, T, src, A);
+  hipExtLaunchKernelGGL((bound_kernel_pk<MM, NW, Src>), dim3(grid), dim3(block), shmem, stream,
+                        A.ev_start, A.ev_stop, 0, T, src, A);
   return hipGetLastError();
 }
 
@@ -854,8 +857,8 @@ hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
   if (grid < 1) grid = 1;
   const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + 4 * block) +
                        static_cast<size_t>(T.n) * 64 * tpb;
-  hipLaunchKernelGGL((bound_kernel<MM, NW, MODE, BLOCK, Src>), dim3(grid), dim3(block), shmem,
-                     stream, T, src, A);
+  hipExtLaunchKernelGGL((bound_kernel<MM, NW, MODE, BLOCK, Src>), dim3(grid), dim3(block), shmem,
+                        stream, A.ev_start, A.ev_stop, 0, T, src, A);
   return hipGetLastError();
 }
 

@@ -337,10 +337,14 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   __shared__ int s_state[4];
   Ctrl* C = S.ctrl;
   const int tid = threadIdx.x;
+  // a round after the search ended (or with nothing to pop) does nothing;
+  // the fallback count is cleared with the others so that the statistics
+  // kernel of such a round adds zero
   if (C->status != kRunning) {
     if (tid == 0) {
       C->n_parents = 0;
       C->n_children = 0;
+      C->n_fb = 0;
     }
     return;
   }
@@ -350,6 +354,7 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
       C->status = kDone;
       C->n_parents = 0;
       C->n_children = 0;
+      C->n_fb = 0;
     }
     return;
   }
@@ -367,6 +372,7 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     select_window<kSelWin / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
   if (tid == 0) {
+    C->n_fb = 0;
     if (np > S.PCAP || nc > S.CCAP) {
       C->status = kOverflow;
       C->n_parents = 0;
@@ -399,7 +405,6 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
         C->stats[kStPairs] += static_cast<long long>(nc) * S.topk;
         C->stats[kStSubset] += nc;
       }
-      C->n_fb = 0;
     }
   }
 }
@@ -859,7 +864,8 @@ struct SearchState {
   int n = 0, CAP = 0, PCAP = 0, CCAP = 0;
   DevBuf mem;
   SArgs args{};
-  Ctrl* host_ctrl = nullptr;  // pinned
+  Ctrl* host_ctrl = nullptr;  // pinned, two copies: batches of rounds alternate
+  hipEvent_t copied[2] = {nullptr, nullptr};  // control copy of each batch landed
   // per-round selector buffers: logged features [5m][CCAP] and labels, BF16
   // feature fragments, tile masks, the fallback list
   DevBuf aux;
@@ -909,6 +915,8 @@ void destroy_search(wsbb_ctx* ctx) {
   ctx->search->aux.release();
   ctx->search->mem.release();
   if (ctx->search->host_ctrl) (void)hipHostFree(ctx->search->host_ctrl);
+  for (hipEvent_t& e : ctx->search->copied)
+    if (e) (void)hipEventDestroy(e);
   delete ctx->search;
   ctx->search = nullptr;
 }
@@ -951,7 +959,9 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
     hip_fail(e, "hipMalloc(search pool)");
     return WSBB_ENOMEM;
   }
-  e = hipHostMalloc(reinterpret_cast<void**>(&S->host_ctrl), sizeof(Ctrl), hipHostMallocDefault);
+  e = hipHostMalloc(reinterpret_cast<void**>(&S->host_ctrl), 2 * sizeof(Ctrl), hipHostMallocDefault);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i)
+    e = hipEventCreateWithFlags(&S->copied[i], hipEventDisableTiming);
   if (e != hipSuccess) {
     destroy_search(ctx);
     return hip_fail(e, "hipHostMalloc(ctrl)");
@@ -1046,7 +1056,9 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
       B.feat = SS->r_feat;
     }
   }
-  if (ev0) (void)hipEventRecord(ev0, st);
+  // the bound kernel's own dispatch carries the timing events
+  B.ev_start = ev0;
+  B.ev_stop = ev1;
   e = launch_bound(ctx->T, src, B, ctx->grid_cap(), st);
   if (e != hipSuccess) return e;
   if (kind == WSBB_LEARNED && selector_margin(ctx) >= 0) {
@@ -1065,7 +1077,6 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     if ((e = launch_bound(ctx->T, ls, Bf, ctx->grid_cap(), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_fallback_stats, dim3(1), dim3(1), 0, st, A);
   }
-  if (ev1) (void)hipEventRecord(ev1, st);
   if (kind != WSBB_LEARNED && log_stride > 0) {
     LogArgs L{};
     L.ctrl = A.ctrl;
@@ -1161,9 +1172,10 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "upload of the saved pool");
   }
-  const bool timing = ctx->profile_bound;
-  if (timing && static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
-    while (static_cast<int>(S->ev.size()) < 2 * rounds_per_sync) {
+  const int every = ctx->profile_every;
+  const bool timing = every > 0;
+  if (timing && static_cast<int>(S->ev.size()) < 4 * rounds_per_sync) {
+    while (static_cast<int>(S->ev.size()) < 4 * rounds_per_sync) {
       hipEvent_t ev;
       if ((e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
       S->ev.push_back(ev);
@@ -1199,30 +1211,61 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
       return hip_fail(e, "hipGraphInstantiate");
     }
   }
-  for (;;) {
+  // Batches of rounds_per_sync rounds, each followed by a copy of the
+  // control block into its own pinned slot.  The next batch is queued before
+  // the host waits on the current one, so the GPU never idles on a host
+  // round trip; once a batch reports the search finished, the batch behind
+  // it only repeats finished rounds (each returns at once) and is drained.
+  auto enqueue = [&](int slot) -> hipError_t {
+    hipError_t le;
     if (use_graph) {
-      if ((e = hipGraphLaunch(S->graph, ctx->stream)) != hipSuccess)
-        return hip_fail(e, "hipGraphLaunch");
+      if ((le = hipGraphLaunch(S->graph, ctx->stream)) != hipSuccess) return le;
     } else {
-      for (int r = 0; r < rounds_per_sync; ++r)
-        if ((e = launch_round(ctx, A, kind, timing ? S->ev[2 * r] : nullptr,
-                              timing ? S->ev[2 * r + 1] : nullptr)) != hipSuccess)
-          return hip_fail(e, "round launch");
-    }
-    e = hipMemcpyAsync(S->host_ctrl, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "search sync");
-    if (timing)
       for (int r = 0; r < rounds_per_sync; ++r) {
-        float t = 0;
-        if (hipEventElapsedTime(&t, S->ev[2 * r], S->ev[2 * r + 1]) == hipSuccess) {
-          S->bound_ms += t;
-          ++S->bound_launches;
-        }
+        hipEvent_t* ev = timing && r % every == 0 ? &S->ev[2 * (slot * rounds_per_sync + r)] : nullptr;
+        if ((le = launch_round(ctx, A, kind, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr)) !=
+            hipSuccess)
+          return le;
       }
-    if (S->host_ctrl->status != kRunning) break;
+    }
+    le = hipMemcpyAsync(S->host_ctrl + slot, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                        ctx->stream);
+    if (le == hipSuccess) le = hipEventRecord(S->copied[slot], ctx->stream);
+    return le;
+  };
+  // rounds of a batch that did work: the first (rounds counter growth) ones
+  long long prev_rounds = (resume && resume->stats) ? resume->stats[kStRounds] : 0;
+  auto collect = [&](int slot) -> hipError_t {
+    const hipError_t le = hipEventSynchronize(S->copied[slot]);
+    if (le != hipSuccess) return le;
+    const long long live = S->host_ctrl[slot].stats[kStRounds] - prev_rounds;
+    prev_rounds = S->host_ctrl[slot].stats[kStRounds];
+    if (!timing) return le;
+    for (int r = 0; r < rounds_per_sync && r < live; r += every) {
+      float t = 0;
+      const int i = 2 * (slot * rounds_per_sync + r);
+      if (hipEventElapsedTime(&t, S->ev[i], S->ev[i + 1]) == hipSuccess) {
+        S->bound_ms += t;
+        ++S->bound_launches;
+      }
+    }
+    return le;
+  };
+  int cur = 0;
+  if ((e = enqueue(cur)) != hipSuccess) return hip_fail(e, "round launch");
+  for (;;) {
+    if ((e = enqueue(cur ^ 1)) != hipSuccess) return hip_fail(e, "round launch");
+    if ((e = collect(cur)) != hipSuccess) return hip_fail(e, "search sync");
+    if (S->host_ctrl[cur].status != kRunning) {
+      // the batch behind repeats finished rounds: wait for it, keep its
+      // (identical) control block and leave its timings out
+      if ((e = hipEventSynchronize(S->copied[cur ^ 1])) != hipSuccess)
+        return hip_fail(e, "search sync");
+      break;
+    }
+    cur ^= 1;
   }
-  const Ctrl& H = *S->host_ctrl;
+  const Ctrl& H = S->host_ctrl[cur];
   if (best) *best = H.inc;
   if (best_perm) {
     for (int i = 0; i < n; ++i) best_perm[i] = H.have_perm ? H.best_perm[i] : -1;
@@ -1248,7 +1291,7 @@ extern "C" int wsbb_search_profile(wsbb_ctx* ctx, int enable, double* bound_ms,
     wsbb::set_error("null context");
     return WSBB_EINVAL;
   }
-  if (enable >= 0) ctx->profile_bound = enable != 0;
+  if (enable >= 0) ctx->profile_every = enable;
   if (bound_ms) *bound_ms = ctx->search ? ctx->search->bound_ms : 0.0;
   if (bound_launches) *bound_launches = ctx->search ? ctx->search->bound_launches : 0;
   return WSBB_OK;
