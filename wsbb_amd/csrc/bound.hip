@@ -29,11 +29,6 @@ namespace wsbb {
 
 namespace {
 
-template <int NW>
-__device__ __forceinline__ void set_job_bit(uint32_t (&sm)[NW], int j) {
-#pragma unroll
-  for (int w = 0; w < NW; ++w) sm[w] |= ((j >> 5) == w) ? (1u << (j & 31)) : 0u;
-}
 
 // float -> bfloat16, round to nearest even (finite inputs)
 __device__ __forceinline__ uint16_t f2bf_dev(float f) {
@@ -94,31 +89,6 @@ __device__ __forceinline__ void step4(const StepGroup<NW>& g, const uint32_t (&u
 // MODE 2: MODE 1 plus LB1 and the selector features; MODE 3: MODE 0 over the
 // pairs of each tile's subset (tile_sel) only.  Modes 0 and 3 write their
 // result to the source's slot of the node.
-// Processing times staged in LDS job-major, [n][MM] with machines m..MM-1
-// zero: the head/tail recurrences read a job's machines as MM/4 16-byte
-// loads from one address instead of MM loads from MM rows.
-template <int MM>
-__device__ __forceinline__ void stage_times(const DevTables& T, int32_t* sp) {
-  static_assert(MM % 4 == 0, "machine rows are read four at a time");
-  const int n = T.n, m = T.m;
-  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) {
-    const int j = i / MM, k = i - j * MM;
-    sp[i] = k < m ? T.p[k * n + j] : 0;
-  }
-}
-template <int MM>
-__device__ __forceinline__ void job_row(const int32_t* sp, int j, int (&pk)[MM]) {
-  const int4* rp = reinterpret_cast<const int4*>(sp + j * MM);
-#pragma unroll
-  for (int k4 = 0; k4 < MM / 4; ++k4) {
-    const int4 v = rp[k4];
-    pk[4 * k4] = v.x;
-    pk[4 * k4 + 1] = v.y;
-    pk[4 * k4 + 2] = v.z;
-    pk[4 * k4 + 3] = v.w;
-  }
-}
-
 template <int MM, int NW, int MODE, int BLOCK, class Src>
 __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, BoundArgs A) {
   constexpr bool LEAN = MODE == 0 || MODE == 3, EXTRA = MODE == 2;
@@ -174,12 +144,16 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
       l2 = n;
     }
     const int d1 = l1 + 1, d2 = n - l2;
-    // the tile's first wave stages it (all loads of a chunk are issued
-    // before the first LDS store waits); the other waves of the tile read it
-    // after the barrier
+    // children of a search round carry their parent's heads, tails and job
+    // mask: a child extends them by the one or two jobs it places, and no
+    // permutation is staged.  Otherwise the tile's first wave stages the
+    // permutations (all loads of a chunk are issued before the first LDS
+    // store waits); the other waves of the tile read them after the barrier.
+    bool from_parent = false;
+    if constexpr (Src::kState && !EXTRA) from_parent = src.has_state();
     const uint8_t* mine = pl + tib * 64 + lane;
     const int bstride = 64 * tpb;
-    if (grp == 0 && valid) {
+    if (!from_parent && grp == 0 && valid) {
       for (int i0 = 0; i0 < n; i0 += 16) {
         uint16_t v[16];
 #pragma unroll
@@ -205,29 +179,29 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
 #pragma unroll
     for (int w = 0; w < NW; ++w) sm[w] = 0;
 
-    // the next job index is read while the current row is processed
-    int jn = d1 > 0 ? perm_at(mine, bstride, sw0, sw1, 0) : 0;
-    for (int i = 0; i < d1; ++i) {
-      const int j = jn;
-      if (i + 1 < d1) jn = perm_at(mine, bstride, sw0, sw1, i + 1);
-      set_job_bit<NW>(sm, j);
-      int pk[MM];
-      job_row<MM>(sp, j, pk);
-      r[0] += pk[0];
-#pragma unroll
-      for (int k = 1; k < MM; ++k)
-        r[k] = max(r[k], r[k - 1]) + pk[k];
-    }
-    jn = d2 > 0 ? perm_at(mine, bstride, sw0, sw1, n - 1) : 0;
-    for (int i = n - 1; i >= l2; --i) {
-      const int j = jn;
-      if (i - 1 >= l2) jn = perm_at(mine, bstride, sw0, sw1, i - 1);
-      set_job_bit<NW>(sm, j);
-      int pk[MM];
-      job_row<MM>(sp, j, pk);
-#pragma unroll
-      for (int k = MM - 1; k >= 0; --k)
-        q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
+    if (from_parent) {
+      if constexpr (Src::kState)
+        if (valid) src.template state<MM, NW>(node, sp, r, q, sm);
+    } else {
+      // the next job index is read while the current row is processed
+      int jn = d1 > 0 ? perm_at(mine, bstride, sw0, sw1, 0) : 0;
+      for (int i = 0; i < d1; ++i) {
+        const int j = jn;
+        if (i + 1 < d1) jn = perm_at(mine, bstride, sw0, sw1, i + 1);
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        head_step<MM>(r, pk);
+      }
+      jn = d2 > 0 ? perm_at(mine, bstride, sw0, sw1, n - 1) : 0;
+      for (int i = n - 1; i >= l2; --i) {
+        const int j = jn;
+        if (i - 1 >= l2) jn = perm_at(mine, bstride, sw0, sw1, i - 1);
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        tail_step<MM>(q, pk);
+      }
     }
     const bool leaf = (l2 - l1 == 1);
     int leafv = 0;
@@ -488,6 +462,9 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
     const int hi = n - 32 * w;
     valid_bits[w] = hi >= 32 ? 0xffffffffu : (hi > 0 ? ((1u << hi) - 1u) : 0u);
   }
+  // the two-node form runs the recurrences itself (parent state, which
+  // bound_kernel uses, measured slower here: two nodes' state per lane)
+  constexpr bool from_parent = false;
   for (int bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
     const int tile = __builtin_amdgcn_readfirstlane(bt * tpb + tib);
     int node[2], l1[2], l2[2], sw0[2], sw1[2];
@@ -505,7 +482,7 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
         l1[h] = -1;
         l2[h] = n;
       }
-      if (grp == 0 && valid[h]) {
+      if (!from_parent && grp == 0 && valid[h]) {
         for (int i0 = 0; i0 < n; i0 += 16) {
           uint16_t v[16];
 #pragma unroll
@@ -533,26 +510,25 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
       for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) sm[w] = 0;
-      int jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
-      for (int i = 0; i < d1; ++i) {
-        const int j = jn;
-        if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
-        set_job_bit<NW>(sm, j);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-        r[0] += pk[0];
-#pragma unroll
-        for (int k = 1; k < MM; ++k) r[k] = max(r[k], r[k - 1]) + pk[k];
-      }
-      jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
-      for (int i = n - 1; i >= l2[h]; --i) {
-        const int j = jn;
-        if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
-        set_job_bit<NW>(sm, j);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-#pragma unroll
-        for (int k = MM - 1; k >= 0; --k) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
+      {
+        int jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
+        for (int i = 0; i < d1; ++i) {
+          const int j = jn;
+          if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
+          set_job_bit<NW>(sm, j);
+          int pk[MM];
+          job_row<MM>(sp, j, pk);
+          head_step<MM>(r, pk);
+        }
+        jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
+        for (int i = n - 1; i >= l2[h]; --i) {
+          const int j = jn;
+          if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
+          set_job_bit<NW>(sm, j);
+          int pk[MM];
+          job_row<MM>(sp, j, pk);
+          tail_step<MM>(q, pk);
+        }
       }
       int lv = 0;
 #pragma unroll

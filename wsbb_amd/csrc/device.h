@@ -74,8 +74,61 @@ struct BoundArgs {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
+// Processing times staged in LDS job-major, [n][MM] with machines m..MM-1
+// zero: the head/tail recurrences read a job's machines as MM/4 16-byte
+// loads from one address instead of MM loads from MM rows.
+template <int MM>
+__device__ __forceinline__ void stage_times(const DevTables& T, int32_t* sp) {
+  static_assert(MM % 4 == 0, "machine rows are read four at a time");
+  const int n = T.n, m = T.m;
+  for (int i = threadIdx.x; i < MM * n; i += blockDim.x) {
+    const int j = i / MM, k = i - j * MM;
+    sp[i] = k < m ? T.p[k * n + j] : 0;
+  }
+}
+template <int MM>
+__device__ __forceinline__ void job_row(const int32_t* sp, int j, int (&pk)[MM]) {
+  const int4* rp = reinterpret_cast<const int4*>(sp + j * MM);
+#pragma unroll
+  for (int k4 = 0; k4 < MM / 4; ++k4) {
+    const int4 v = rp[k4];
+    pk[4 * k4] = v.x;
+    pk[4 * k4 + 1] = v.y;
+    pk[4 * k4 + 2] = v.z;
+    pk[4 * k4 + 3] = v.w;
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ void set_job_bit(uint32_t (&sm)[NW], int j) {
+#pragma unroll
+  for (int w = 0; w < NW; ++w) sm[w] |= ((j >> 5) == w) ? (1u << (j & 31)) : 0u;
+}
+
+// Head/tail recurrences by one job (rows m..MM-1 of the times are zero, so
+// the padded machines need no guards): heads r of a prefix extended by job j
+// at its end, tails q of a suffix extended by job j at its front.
+template <int MM>
+__device__ __forceinline__ void head_step(int (&r)[MM], const int (&pk)[MM]) {
+  r[0] += pk[0];
+#pragma unroll
+  for (int k = 1; k < MM; ++k) r[k] = max(r[k], r[k - 1]) + pk[k];
+}
+template <int MM>
+__device__ __forceinline__ void tail_step(int (&q)[MM], const int (&pk)[MM]) {
+#pragma unroll
+  for (int k = MM - 1; k >= 0; --k) q[k] = max(q[k], (k + 1 < MM) ? q[k + 1] : 0) + pk[k];
+}
+
+__device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,
+                                       int sw1, int i) {
+  const int src = (i == sw0) ? sw1 : ((i == sw1) ? sw0 : i);
+  return col[static_cast<size_t>(src) * stride];
+}
+
 // Node source: explicit position-major nodes.
 struct BatchSource {
+  static constexpr bool kState = false;  // no parent state: heads/tails from the permutation
   const uint8_t* permT;  // [n][N]
   const int32_t* l1;
   const int32_t* l2;
@@ -96,11 +149,53 @@ struct BatchSource {
 // Node source: children described by (parent, side, position) over a
 // position-major parent array (decompose_beginend child semantics).
 struct ChildSource {
+  static constexpr bool kState = true;
   const uint8_t* ppermT;  // parents [n][PN]
   const int32_t* pl1;
   const int32_t* pl2;
   const int32_t* desc;    // [N] parent << 9 | side << 8 | j
   int PN, N;
+  // optional parent state (heads [MM][PN], tails [MM][PN] of the parent's
+  // prefix and suffix, scheduled-job masks [NW][PN]): a child's heads and
+  // tails are then its parent's extended by the one or two jobs it places
+  const int32_t* hr = nullptr;
+  const int32_t* hq = nullptr;
+  const uint32_t* hm = nullptr;
+  __device__ __forceinline__ bool has_state() const { return hr != nullptr; }
+  // heads, tails and scheduled mask of child k from its parent's state; sp
+  // holds the processing times job-major in LDS
+  template <int MM, int NW>
+  __device__ __forceinline__ void state(int k, const int32_t* sp, int (&r)[MM], int (&q)[MM],
+                                        uint32_t (&sm)[NW]) const {
+    const int d = desc[k];
+    const int par = d >> 9, side = (d >> 8) & 1, j = d & 255;
+    const int pa = pl1[par], pb = pl2[par];
+#pragma unroll
+    for (int kk = 0; kk < MM; ++kk) {
+      r[kk] = hr[static_cast<size_t>(kk) * PN + par];
+      q[kk] = hq[static_cast<size_t>(kk) * PN + par];
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sm[w] = hm[static_cast<size_t>(w) * PN + par];
+    const uint8_t* col = ppermT + par;
+    const int job = col[static_cast<size_t>(j) * PN];  // placed at l1 + 1 or l2 - 1
+    set_job_bit<NW>(sm, job);
+    int pk[MM];
+    job_row<MM>(sp, job, pk);
+    if (side == 0)
+      head_step<MM>(r, pk);
+    else
+      tail_step<MM>(q, pk);
+    // the last free job goes to the end of the prefix (limites_set)
+    const int a = pa + (side == 0), b = pb - (side == 1);
+    if (b - a == 2) {
+      const int sw0 = side == 0 ? pa + 1 : pb - 1;
+      const int j2 = perm_at(col, PN, sw0, j, a + 1);
+      set_job_bit<NW>(sm, j2);
+      job_row<MM>(sp, j2, pk);
+      head_step<MM>(r, pk);
+    }
+  }
   __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
                                        int& stride, int& sw0, int& sw1) const {
     if (k >= N) return false;
@@ -123,9 +218,16 @@ struct ChildSource {
 // Node source: the children listed in list[0..*count) (a compacted subset of
 // a round's children); results go to the listed child's slot.
 struct ListSource {
+  static constexpr bool kState = true;
   ChildSource base;
   const int32_t* list;
   const int* count;
+  __device__ __forceinline__ bool has_state() const { return base.has_state(); }
+  template <int MM, int NW>
+  __device__ __forceinline__ void state(int k, const int32_t* sp, int (&r)[MM], int (&q)[MM],
+                                        uint32_t (&sm)[NW]) const {
+    base.state<MM, NW>(list[k], sp, r, q, sm);
+  }
   __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
                                        int& stride, int& sw0, int& sw1) const {
     if (k >= *count) return false;
@@ -134,10 +236,5 @@ struct ListSource {
   __device__ __forceinline__ int slot(int k) const { return list[k]; }
 };
 
-__device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,
-                                       int sw1, int i) {
-  const int src = (i == sw0) ? sw1 : ((i == sw1) ? sw0 : i);
-  return col[static_cast<size_t>(src) * stride];
-}
 
 }  // namespace wsbb

@@ -98,6 +98,9 @@ struct SArgs {
   int* c_bound;
   int* c_rank;
   int* c_scost;
+  int* p_hr;        // [MM][PCAP + 1] heads of the parent's prefix
+  int* p_hq;        // [MM][PCAP + 1] tails of the parent's suffix
+  uint32_t* p_hm;   // [NW][PCAP + 1] the parent's scheduled jobs
 };
 
 struct OpAdd {
@@ -411,14 +414,24 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
 
 // ---- expand: parents out of the stack, child descriptors ------------------
 // A thread per parent copies its permutation column (consecutive parents sit
-// at consecutive stack slots, so every position row is read coalesced) and
-// writes its children's descriptors.  Written for latency (a round has a few
+// at consecutive stack slots, so every position row is read coalesced),
+// writes its children's descriptors and computes the parent's heads, tails
+// and scheduled-job mask once: the bound kernel extends them by the one or
+// two jobs each child places instead of rerunning the recurrences over the
+// whole prefix and suffix per child.  Written for latency (a round has a few
 // thousand parents): one batch of loads per column chunk, no lane exchanges.
-__global__ __launch_bounds__(256) void k_expand(SArgs S) {
+template <int MM, int NW>
+__global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_state) {
+  extern __shared__ int32_t lds[];
+  int32_t* sp = lds;                                          // [n][MM] job-major
+  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * T.n);    // [n][256] own columns
+  stage_times<MM>(T, sp);
+  __syncthreads();
   const Ctrl* C = S.ctrl;
   const int np = C->n_parents;
   const int n = S.n, m = S.m, P = S.P;
   const int lane = threadIdx.x & 63;
+  const size_t PH = static_cast<size_t>(S.PCAP);  // rows of the parent state, as ChildSource reads them
   long long work = 0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
     const int si = S.p_sel[p];
@@ -432,7 +445,10 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S) {
         if (i0 + u < n) v[u] = S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + si];
 #pragma unroll
       for (int u = 0; u < 32; ++u)
-        if (i0 + u < n) S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
+        if (i0 + u < n) {
+          S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
+          pl[(i0 + u) * 256 + threadIdx.x] = v[u];
+        }
     }
     S.p_l1[p] = l1;
     S.p_l2[p] = l2;
@@ -452,11 +468,66 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S) {
       const long long base = P * (n + 4 * (u - 1LL) + 4);
       work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
     }
+    if (!want_state) continue;
+    // the parent's heads and tails (no R0/Q0 substitution: that depends on
+    // the child's own prefix and suffix) and scheduled jobs
+    int r[MM], q[MM];
+    uint32_t sm[NW];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sm[w] = 0;
+    const uint8_t* mine = pl + threadIdx.x;
+    int jn = l1 >= 0 ? mine[0] : 0;
+    for (int i = 0; i <= l1; ++i) {
+      const int j = jn;
+      if (i + 1 <= l1) jn = mine[(i + 1) * 256];
+      set_job_bit<NW>(sm, j);
+      int pk[MM];
+      job_row<MM>(sp, j, pk);
+      head_step<MM>(r, pk);
+    }
+    jn = l2 < n ? mine[(n - 1) * 256] : 0;
+    for (int i = n - 1; i >= l2; --i) {
+      const int j = jn;
+      if (i - 1 >= l2) jn = mine[(i - 1) * 256];
+      set_job_bit<NW>(sm, j);
+      int pk[MM];
+      job_row<MM>(sp, j, pk);
+      tail_step<MM>(q, pk);
+    }
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      S.p_hr[k * PH + p] = r[k];
+      S.p_hq[k * PH + p] = q[k];
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) S.p_hm[w * PH + p] = sm[w];
   }
   for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
   if (lane == 0 && work)
     atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStWork]),
               static_cast<unsigned long long>(work));
+}
+
+hipError_t launch_expand(const wsbb_ctx* ctx, const SArgs& A, int want_state, hipStream_t st) {
+  const int m = ctx->T.m, n = ctx->T.n;
+  const int mm = m <= 8 ? 8 : (m <= 12 ? 12 : (m <= 16 ? 16 : (m <= 20 ? 20 : 24)));
+  const int nw = n <= 32 ? 1 : (n <= 64 ? 2 : 4);
+  const dim3 grid((A.PCAP + 255) / 256), block(256);
+  const size_t shmem = sizeof(int32_t) * mm * n + static_cast<size_t>(n) * 256;
+#define WSBB_EXPAND(MMv, NWv)                                                                \
+  if (mm == MMv && nw == NWv) {                                                              \
+    hipLaunchKernelGGL((k_expand<MMv, NWv>), grid, block, shmem, st, A, ctx->T, want_state); \
+    return hipGetLastError();                                                                \
+  }
+  WSBB_EXPAND(8, 1) WSBB_EXPAND(8, 2) WSBB_EXPAND(8, 4)
+  WSBB_EXPAND(12, 1) WSBB_EXPAND(12, 2) WSBB_EXPAND(12, 4)
+  WSBB_EXPAND(16, 1) WSBB_EXPAND(16, 2) WSBB_EXPAND(16, 4)
+  WSBB_EXPAND(20, 1) WSBB_EXPAND(20, 2) WSBB_EXPAND(20, 4)
+  WSBB_EXPAND(24, 1) WSBB_EXPAND(24, 2) WSBB_EXPAND(24, 4)
+#undef WSBB_EXPAND
+  return hipErrorInvalidValue;
 }
 
 // ---- rank: one wave per parent -------------------------------------------
@@ -953,6 +1024,13 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   for (int i = 0; i < 13; ++i) o_pint[i] = take(4 * size_t(PCAP + 1 + kSettleK));
   size_t o_cint[4];
   for (int i = 0; i < 4; ++i) o_cint[i] = take(4 * size_t(CCAP));
+  // parent state rows: padded machine count, job-mask words
+  const int m = ctx->T.m;
+  const int mmp = m <= 8 ? 8 : (m <= 12 ? 12 : (m <= 16 ? 16 : (m <= 20 ? 20 : 24)));
+  const int nwp = n <= 32 ? 1 : (n <= 64 ? 2 : 4);
+  const size_t o_hr = take(4 * size_t(mmp) * (PCAP + 1));
+  const size_t o_hq = take(4 * size_t(mmp) * (PCAP + 1));
+  const size_t o_hm = take(4 * size_t(nwp) * (PCAP + 1));
   hipError_t e = S->mem.reserve(off);
   if (e != hipSuccess) {
     destroy_search(ctx);
@@ -985,18 +1063,32 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   for (int i = 0; i < 13; ++i) *pints[i] = S->mem.as<int>(o_pint[i]);
   int** cints[4] = {&A.c_desc, &A.c_bound, &A.c_rank, &A.c_scost};
   for (int i = 0; i < 4; ++i) *cints[i] = S->mem.as<int>(o_cint[i]);
+  A.p_hr = S->mem.as<int>(o_hr);
+  A.p_hq = S->mem.as<int>(o_hq);
+  A.p_hm = S->mem.as<uint32_t>(o_hm);
   return WSBB_OK;
 }
 
 static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent_t ev0,
                                hipEvent_t ev1) {
   hipStream_t st = ctx->stream;
-  const int eblocks = (A.PCAP + 255) / 256;
   const int cblocks = (A.CCAP + 255) / 256;
   const int rblocks = (A.PCAP + 3) / 4;
   hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
-  hipLaunchKernelGGL(k_expand, dim3(eblocks), dim3(256), 0, st, A);
+  // parent state for the int32 bound kernel (learned subsets, logging,
+  // rounds below the two-node form's batch size); the two-node form of plain
+  // LB2 runs its own recurrences
+  const bool pk_round = kind == WSBB_LB2 && selector_log_stride(ctx) == 0 &&
+                        ctx->T.steps16 != nullptr && ctx->int16_for(A.CCAP);
+  const bool want_state = !pk_round;
+  hipError_t e = launch_expand(ctx, A, want_state ? 1 : 0, st);
+  if (e != hipSuccess) return e;
   ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
+  if (want_state) {
+    src.hr = A.p_hr;
+    src.hq = A.p_hq;
+    src.hm = A.p_hm;
+  }
   SearchState* SS = ctx->search;
   const int log_stride = selector_log_stride(ctx);
   BoundArgs B{};
@@ -1017,7 +1109,6 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   B.out = A.c_bound;
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
   B.pk = ctx->int16_for(A.CCAP);
-  hipError_t e;
   const bool per_parent = selector_parents(ctx);
   const BatchSource parents{A.p_perm, A.p_l1, A.p_l2, A.PCAP};
   const float fscale = static_cast<float>(ctx->inst.path_bound);
