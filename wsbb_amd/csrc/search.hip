@@ -101,6 +101,12 @@ struct SArgs {
   int* p_hr;        // [MM][PCAP + 1] heads of the parent's prefix
   int* p_hq;        // [MM][PCAP + 1] tails of the parent's suffix
   uint32_t* p_hm;   // [NW][PCAP + 1] the parent's scheduled jobs
+  // selector features of the parents written by expand (m a multiple of
+  // four): BF16 MFMA fragments and/or FP32 rows [5m][x_ld], or null
+  uint2* x_frag;
+  float* x_f32;
+  int x_ld, x_fsteps;
+  float x_inv_scale;
 };
 
 struct OpAdd {
@@ -470,11 +476,11 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
     }
     if (!want_state) continue;
     // the parent's heads and tails (no R0/Q0 substitution: that depends on
-    // the child's own prefix and suffix) and scheduled jobs
-    int r[MM], q[MM];
+    // the child's own prefix and suffix), scheduled jobs and their work
+    int r[MM], q[MM], wk[MM];
     uint32_t sm[NW];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
+    for (int k = 0; k < MM; ++k) r[k] = q[k] = wk[k] = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) sm[w] = 0;
     const uint8_t* mine = pl + threadIdx.x;
@@ -486,6 +492,8 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
       int pk[MM];
       job_row<MM>(sp, j, pk);
       head_step<MM>(r, pk);
+#pragma unroll
+      for (int k = 0; k < MM; ++k) wk[k] += pk[k];
     }
     jn = l2 < n ? mine[(n - 1) * 256] : 0;
     for (int i = n - 1; i >= l2; --i) {
@@ -495,6 +503,8 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
       int pk[MM];
       job_row<MM>(sp, j, pk);
       tail_step<MM>(q, pk);
+#pragma unroll
+      for (int k = 0; k < MM; ++k) wk[k] += pk[k];
     }
 #pragma unroll
     for (int k = 0; k < MM; ++k) {
@@ -503,6 +513,41 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
     }
 #pragma unroll
     for (int w = 0; w < NW; ++w) S.p_hm[w * PH + p] = sm[w];
+    // selector features of the parent (the feature kernel's definition and
+    // fragment layout; used with m == MM so every index is a constant): per
+    // machine head (or R0), tail (or Q0), remaining work, head + work,
+    // work + tail, scaled
+    if (S.x_frag != nullptr || S.x_f32 != nullptr) {
+      int h[MM], tl[MM], rw[MM];
+#pragma unroll
+      for (int k = 0; k < MM; ++k) {
+        h[k] = l1 >= 0 ? r[k] : ld_const(T.R0, k);
+        tl[k] = l2 < n ? q[k] : ld_const(T.Q0, k);
+        rw[k] = ld_const(T.ptot, k) - wk[k];
+      }
+      const int b = p >> 5, c = p & 31;
+#pragma unroll
+      for (int t = 0; t < (5 * MM + 7) / 8; ++t)
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          uint32_t w2[2] = {0u, 0u};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 8 * t + 4 * half + i, g = f / MM, k = f % MM;
+            if (f < 5 * MM) {
+              const int v = g == 0 ? h[k] : g == 1 ? tl[k] : g == 2 ? rw[k] : g == 3 ? h[k] + rw[k] : rw[k] + tl[k];
+              const float x = static_cast<float>(v) * S.x_inv_scale;
+              if (S.x_f32) S.x_f32[static_cast<size_t>(f) * S.x_ld + p] = x;
+              uint32_t u = __float_as_uint(x);
+              u += 0x7FFFu + ((u >> 16) & 1u);  // BF16, round to nearest even
+              w2[i >> 1] |= (u >> 16) << (16 * (i & 1));
+            }
+          }
+          if (S.x_frag)
+            S.x_frag[(static_cast<size_t>(b) * S.x_fsteps + t) * 64 + half * 32 + c] =
+                make_uint2(w2[0], w2[1]);
+        }
+    }
   }
   for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
   if (lane == 0 && work)
@@ -1081,7 +1126,21 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   const bool pk_round = kind == WSBB_LB2 && selector_log_stride(ctx) == 0 &&
                         ctx->T.steps16 != nullptr && ctx->int16_for(A.CCAP);
   const bool want_state = !pk_round;
-  hipError_t e = launch_expand(ctx, A, want_state ? 1 : 0, st);
+  // per-parent selector features come from expand when m is a multiple of
+  // four (its constant-index form), else from the feature kernel
+  SArgs Ax = A;
+  const bool per_parent = selector_parents(ctx);
+  const bool feat_in_expand = per_parent && ctx->T.m % 4 == 0 &&
+                              (kind == WSBB_LEARNED || selector_log_stride(ctx) > 0);
+  if (feat_in_expand) {
+    SearchState* S0 = ctx->search;
+    Ax.x_frag = kind == WSBB_LEARNED ? S0->r_xfrag : nullptr;
+    Ax.x_f32 = kind == WSBB_LEARNED ? nullptr : S0->r_feat;
+    Ax.x_ld = A.PCAP;
+    Ax.x_fsteps = selector_fsteps(ctx);
+    Ax.x_inv_scale = 1.0f / static_cast<float>(ctx->inst.path_bound);
+  }
+  hipError_t e = launch_expand(ctx, Ax, (want_state || feat_in_expand) ? 1 : 0, st);
   if (e != hipSuccess) return e;
   ChildSource src{A.p_perm, A.p_l1, A.p_l2, A.c_desc, A.PCAP, A.CCAP};
   if (want_state) {
@@ -1109,7 +1168,6 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   B.out = A.c_bound;
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
   B.pk = ctx->int16_for(A.CCAP);
-  const bool per_parent = selector_parents(ctx);
   const BatchSource parents{A.p_perm, A.p_l1, A.p_l2, A.PCAP};
   const float fscale = static_cast<float>(ctx->inst.path_bound);
   if (kind == WSBB_LEARNED) {
@@ -1118,7 +1176,9 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     // parents are scored and each tile sums its parents' probabilities.
     if (per_parent) {
       FeatArgs Fa{A.PCAP, &A.ctrl->n_parents, fscale, selector_fsteps(ctx), SS->r_xfrag};
-      if ((e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+      if (!feat_in_expand &&
+          (e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess)
+        return e;
       if ((e = launch_mlp_select(ctx, SS->r_xfrag, A.PCAP, &A.ctrl->n_parents, nullptr, nullptr, 0,
                                  nullptr, SS->r_prob)) != hipSuccess)
         return e;
@@ -1142,7 +1202,9 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     if (per_parent) {
       FeatArgs Fa{A.PCAP, &A.ctrl->n_parents, fscale, selector_fsteps(ctx), nullptr,
                   SS->r_feat, A.PCAP};
-      if ((e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess) return e;
+      if (!feat_in_expand &&
+          (e = launch_features(ctx->T, parents, Fa, ctx->grid_cap(), st)) != hipSuccess)
+        return e;
     } else {
       B.feat = SS->r_feat;
     }

@@ -94,6 +94,13 @@ __device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
 template <int PB, int OUT>
 __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   extern __shared__ uint2 s_w[];
+  // the grid is sized for the largest batch: blocks without work leave
+  // before staging the weights
+  {
+    const int N0 = A.Ndev ? *A.Ndev : A.N;
+    const int nw0 = A.tile_list ? A.nlist : OUT == 2 ? (N0 + 31) >> 5 : (N0 + 63) >> 6;
+    if (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x >> 6) >= nw0) return;
+  }
   const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
   // 16 loads in flight per thread before their LDS stores
   for (int i0 = 0; i0 < n1 + n2; i0 += 16 * blockDim.x) {
