@@ -253,19 +253,28 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
   // coalesced loads: candidate i = t + 1024 k is the i-th from the top; its
   // reservation w (push_back1) and children c share one LDS word (both
   // at most 2n)
+  // every load is issued before the first use (indices past the window are
+  // clamped to a valid slot and their words zeroed), in groups of eight
 #pragma unroll
-  for (int k = 0; k < KPER; ++k) {
-    const int i = tid + 1024 * k;
-    int w = 0, c = 0;
-    if (i < win) {
-      const int si = top - 1 - (st.consumed + i);
-      const int l1 = S.s_l1[si], l2 = S.s_l2[si];
-      const int depth = (l1 + 1) + (n - l2);
-      const int u = n - depth;
-      w = (S.s_cost[si] < inc) ? 2 * u : 0;  // reservation of push_back1
-      c = (depth == n - 2) ? 2 : 2 * u;      // children actually generated
+  for (int k0 = 0; k0 < KPER; k0 += 8) {
+    int a1[8], a2[8], co[8];
+#pragma unroll
+    for (int k = 0; k < 8 && k0 + k < KPER; ++k) {
+      const int i = tid + 1024 * (k0 + k);
+      const int si = top - 1 - (st.consumed + min(i, win - 1));
+      a1[k] = S.s_l1[si];
+      a2[k] = S.s_l2[si];
+      co[k] = S.s_cost[si];
     }
-    s_p[sel_pad(i)] = w | (c << 16);
+#pragma unroll
+    for (int k = 0; k < 8 && k0 + k < KPER; ++k) {
+      const int i = tid + 1024 * (k0 + k);
+      const int depth = (a1[k] + 1) + (n - a2[k]);
+      const int u = n - depth;
+      const int w = (co[k] < inc) ? 2 * u : 0;  // reservation of push_back1
+      const int c = (depth == n - 2) ? 2 : 2 * u;  // children actually generated
+      s_p[sel_pad(i)] = i < win ? (w | (c << 16)) : 0;
+    }
   }
   __syncthreads();
   // blocked inclusive scan: thread t owns candidates [KPER t, KPER t + KPER)
@@ -426,7 +435,7 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
 // two jobs each child places instead of rerunning the recurrences over the
 // whole prefix and suffix per child.  Written for latency (a round has a few
 // thousand parents): one batch of loads per column chunk, no lane exchanges.
-template <int MM, int NW>
+template <int MM, int NW, bool FEAT>
 __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_state) {
   extern __shared__ int32_t lds[];
   int32_t* sp = lds;                                          // [n][MM] job-major
@@ -445,15 +454,15 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
     // loads of a chunk all issue before its stores (the arrays may alias as
     // far as the compiler knows)
     for (int i0 = 0; i0 < n; i0 += 32) {
-      uint8_t v[32];
+      uint32_t v[32];  // one register per byte: every load in flight at once
 #pragma unroll
       for (int u = 0; u < 32; ++u)
         if (i0 + u < n) v[u] = S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + si];
 #pragma unroll
       for (int u = 0; u < 32; ++u)
         if (i0 + u < n) {
-          S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = v[u];
-          pl[(i0 + u) * 256 + threadIdx.x] = v[u];
+          S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = static_cast<uint8_t>(v[u]);
+          pl[(i0 + u) * 256 + threadIdx.x] = static_cast<uint8_t>(v[u]);
         }
     }
     S.p_l1[p] = l1;
@@ -517,7 +526,7 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
     // fragment layout; used with m == MM so every index is a constant): per
     // machine head (or R0), tail (or Q0), remaining work, head + work,
     // work + tail, scaled
-    if (S.x_frag != nullptr || S.x_f32 != nullptr) {
+    if constexpr (FEAT) {
       int h[MM], tl[MM], rw[MM];
 #pragma unroll
       for (int k = 0; k < MM; ++k) {
@@ -561,9 +570,17 @@ hipError_t launch_expand(const wsbb_ctx* ctx, const SArgs& A, int want_state, hi
   const int nw = n <= 32 ? 1 : (n <= 64 ? 2 : 4);
   const dim3 grid((A.PCAP + 255) / 256), block(256);
   const size_t shmem = sizeof(int32_t) * mm * n + static_cast<size_t>(n) * 256;
+  // feature output only where the constant-index form applies (m == MM)
+  const bool feat = (A.x_frag != nullptr || A.x_f32 != nullptr);
+  if (feat && m != mm) return hipErrorInvalidValue;
 #define WSBB_EXPAND(MMv, NWv)                                                                \
   if (mm == MMv && nw == NWv) {                                                              \
-    hipLaunchKernelGGL((k_expand<MMv, NWv>), grid, block, shmem, st, A, ctx->T, want_state); \
+    if (feat)                                                                                \
+      hipLaunchKernelGGL((k_expand<MMv, NWv, true>), grid, block, shmem, st, A, ctx->T,      \
+                         want_state);                                                        \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_expand<MMv, NWv, false>), grid, block, shmem, st, A, ctx->T,     \
+                         want_state);                                                        \
     return hipGetLastError();                                                                \
   }
   WSBB_EXPAND(8, 1) WSBB_EXPAND(8, 2) WSBB_EXPAND(8, 4)
@@ -857,19 +874,20 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
   const int n = S.n;
   const int base = C->push_base, total = C->push_total;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    // the child's and its parent's fields load in two rounds, whatever the
+    // child's fate
     const int rk = S.c_rank[c];
-    if (rk < 0) continue;
     const int d = S.c_desc[c];
     const int p = d >> 9;
-    if (rk >= S.p_kept[p]) continue;
+    const int kept = S.p_kept[p], koff = S.p_koff[p], pl1 = S.p_l1[p], pl2 = S.p_l2[p];
+    if (rk < 0 || rk >= kept) continue;
     const int side = (d >> 8) & 1, j = d & 255;
-    const int slot = base + total - 1 - (S.p_koff[p] + rk);
-    const int pl1 = S.p_l1[p], pl2 = S.p_l2[p];
+    const int slot = base + total - 1 - (koff + rk);
     int l1 = pl1 + (side == 0), l2 = pl2 - (side == 1);
     if (l2 - l1 == 2) l1 += 1;
     const int sw0 = side == 0 ? pl1 + 1 : pl2 - 1;
     for (int i0 = 0; i0 < n; i0 += 16) {
-      uint8_t v[16];
+      uint32_t v[16];  // one register per byte: every load in flight at once
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int i = i0 + u;
@@ -878,7 +896,7 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u)
-        if (i0 + u < n) S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + slot] = v[u];
+        if (i0 + u < n) S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + slot] = static_cast<uint8_t>(v[u]);
     }
     S.s_l1[slot] = l1;
     S.s_l2[slot] = l2;
