@@ -155,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     const int bstride = 64 * tpb;
     if (!from_parent && grp == 0 && valid) {
       for (int i0 = 0; i0 < n; i0 += 16) {
-        uint16_t v[16];
+        uint32_t v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u)
           if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
       }
       if (!from_parent && grp == 0 && valid[h]) {
         for (int i0 = 0; i0 < n; i0 += 16) {
-          uint16_t v[16];
+          uint32_t v[16];
 #pragma unroll
           for (int u = 0; u < 16; ++u)
             if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];

@@ -66,7 +66,8 @@ struct MlpArgs {
   const int* tile_cnt;      // [ntiles] nodes per tile (with tile_list), or null
   const uint2* xfrag;       // [2 ntiles][fstride][64] feature B fragments
   const uint2* frag1;       // [4][fsteps][64] A fragments of W1^T
-  const uint2* frag2;       // [PB][16][64]    A fragments of W2^T
+  const uint2* frag2;       // [PB][16][64]    A fragments of W2^T, stored right
+                            // after frag1 (one allocation)
   const float* b1;          // [128]
   const float* b2;          // [PB*32], padding = -inf
   uint32_t* tile_sel;       // [ntiles][nsw]
@@ -102,13 +103,15 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
     if (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x >> 6) >= nw0) return;
   }
   const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
-  // 16 loads in flight per thread before their LDS stores
+  // 16 loads in flight per thread before their LDS stores: the layer-2
+  // fragments follow the layer-1 ones in memory, and indices past the end
+  // are clamped (their values are not stored)
   for (int i0 = 0; i0 < n1 + n2; i0 += 16 * blockDim.x) {
     uint2 v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int i = i0 + u * blockDim.x + threadIdx.x;
-      if (i < n1 + n2) v[u] = i < n1 ? A.frag1[i] : A.frag2[i - n1];
+      v[u] = A.frag1[min(i, n1 + n2 - 1)];
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {

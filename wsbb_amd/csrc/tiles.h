@@ -76,7 +76,9 @@ __device__ __forceinline__ void parent_tile_mask(const uint16_t* probs, int ldp,
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
         const int s = 64 * i + lane;
-        x[j][i] = s < ldp ? __uint_as_float(static_cast<uint32_t>(row[s]) << 16) : 0.f;
+        // clamped index: every load of the batch is issued before the sums
+        const float v = __uint_as_float(static_cast<uint32_t>(row[min(s, ldp - 1)]) << 16);
+        x[j][i] = s < ldp ? v : 0.f;
       }
     }
 #pragma unroll
