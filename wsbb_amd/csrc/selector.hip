@@ -376,7 +376,79 @@ struct TrainArgs {
   float* gW2;
   float* gb2;
   int nvalid;              // rows with a label (>= 0)
+  float* part;             // split-K partial sums of the weight gradients
 };
+
+// Weight gradients are batch reductions C[m][n] = sum_b A[b][m] Bm[b][n]
+// (A = activations or inputs, Bm = output gradients), with an implicit row
+// m = M of ones in A so that C[M][n] is the bias gradient.  64 x 64 output
+// tiles, the batch split into chunks of kWgradK rows (one block each) that
+// stage 32-row slices of A and Bm in LDS; each thread accumulates a 4 x 4
+// block.  Partial sums go to part[split][M + 1][N] and are added in split
+// order by k_tr_wreduce, so the result does not depend on timing.
+constexpr int kWgradK = 256;
+
+__global__ __launch_bounds__(256) void k_tr_wgrad(const float* A, int M, const float* Bm, int N,
+                                                  int K, float* part) {
+  __shared__ float As[32][68];
+  __shared__ float Bs[32][68];
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64, split = blockIdx.z;
+  const int kb = split * kWgradK, ke = min(kb + kWgradK, K);
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = t + 256 * r, k = e >> 6, c = e & 63;
+      const int kk = k0 + k, m = m0 + c, n = n0 + c;
+      const bool kin = kk < ke;
+      As[k][c] = !kin ? 0.f : m < M ? A[static_cast<size_t>(kk) * M + m] : (m == M ? 1.f : 0.f);
+      Bs[k][c] = kin && n < N ? Bm[static_cast<size_t>(kk) * N + n] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < 32; ++k) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[k][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[k][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+  float* out = part + static_cast<size_t>(split) * (M + 1) * N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + ty * 4 + i;
+    if (m > M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + tx * 4 + j;
+      if (n < N) out[static_cast<size_t>(m) * N + n] = acc[i][j];
+    }
+  }
+}
+
+// C = sum over splits; rows 0..M-1 to W, row M to the bias gradient
+__global__ void k_tr_wreduce(const float* part, int M, int N, int splits, float* W, float* bias) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tot = (M + 1) * N;
+  if (i >= tot) return;
+  float g = 0.f;
+  for (int sp = 0; sp < splits; ++sp) g += part[static_cast<size_t>(sp) * tot + i];
+  if (i < M * N)
+    W[i] = g;
+  else
+    bias[i - M * N] = g;
+}
 
 __global__ void k_tr_in(TrainArgs T) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -405,39 +477,27 @@ __global__ void k_tr_out(TrainArgs T) {
   T.Z[i] = z + T.b2[s];
 }
 
-__global__ void k_tr_softmax(TrainArgs T) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// one wave per row: lanes over the pairs, wave reductions for max and sum
+__global__ __launch_bounds__(256) void k_tr_softmax(TrainArgs T) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (b >= T.B) return;
   const int y = T.label[T.idx[b]];
   const float* z = T.Z + static_cast<size_t>(b) * T.P;
   float* dz = T.dZ + static_cast<size_t>(b) * T.P;
   if (y < 0) {
-    for (int s = 0; s < T.P; ++s) dz[s] = 0.f;
-    T.loss[b] = 0.f;
+    for (int s = lane; s < T.P; s += 64) dz[s] = 0.f;
+    if (lane == 0) T.loss[b] = 0.f;
     return;
   }
   float mx = -INFINITY;
-  for (int s = 0; s < T.P; ++s) mx = fmaxf(mx, z[s]);
+  for (int s = lane; s < T.P; s += 64) mx = fmaxf(mx, z[s]);
+  for (int d = 32; d >= 1; d >>= 1) mx = fmaxf(mx, __shfl_xor(mx, d, 64));
   float sum = 0.f;
-  for (int s = 0; s < T.P; ++s) sum += expf(z[s] - mx);
+  for (int s = lane; s < T.P; s += 64) sum += expf(z[s] - mx);
+  for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
   const float inv = 1.f / sum, scale = 1.f / static_cast<float>(T.nvalid);
-  for (int s = 0; s < T.P; ++s) dz[s] = (expf(z[s] - mx) * inv - (s == y ? 1.f : 0.f)) * scale;
-  T.loss[b] = (logf(sum) + mx - z[y]) * scale;
-}
-
-__global__ void k_tr_gW2(TrainArgs T) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kHidden * T.P) {
-    const int h = i / T.P, s = i - h * T.P;
-    float g = 0.f;
-    for (int b = 0; b < T.B; ++b) g += T.Hb[b * kHidden + h] * T.dZ[static_cast<size_t>(b) * T.P + s];
-    T.gW2[i] = g;
-  } else if (i < kHidden * T.P + T.P) {
-    const int s = i - kHidden * T.P;
-    float g = 0.f;
-    for (int b = 0; b < T.B; ++b) g += T.dZ[static_cast<size_t>(b) * T.P + s];
-    T.gb2[s] = g;
-  }
+  for (int s = lane; s < T.P; s += 64) dz[s] = (expf(z[s] - mx) * inv - (s == y ? 1.f : 0.f)) * scale;
+  if (lane == 0) T.loss[b] = (logf(sum) + mx - z[y]) * scale;
 }
 
 __global__ void k_tr_dH(TrainArgs T) {
@@ -448,21 +508,6 @@ __global__ void k_tr_dH(TrainArgs T) {
   if (T.A1[i] > 0.f)
     for (int s = 0; s < T.P; ++s) g += T.dZ[static_cast<size_t>(b) * T.P + s] * bfround(T.W2[h * T.P + s]);
   T.dH[i] = g;
-}
-
-__global__ void k_tr_gW1(TrainArgs T) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < T.F * kHidden) {
-    const int f = i / kHidden, h = i - f * kHidden;
-    float g = 0.f;
-    for (int b = 0; b < T.B; ++b) g += T.X[b * T.F + f] * T.dH[b * kHidden + h];
-    T.gW1[i] = g;
-  } else if (i < T.F * kHidden + kHidden) {
-    const int h = i - T.F * kHidden;
-    float g = 0.f;
-    for (int b = 0; b < T.B; ++b) g += T.dH[b * kHidden + h];
-    T.gb1[h] = g;
-  }
 }
 
 __global__ void k_adam(float* w, float* m, float* v, const float* g, int n, float lr,
@@ -710,10 +755,15 @@ void selector_log_buffers(const wsbb_ctx* ctx, float** feat, int** label, int** 
   *cap = S->log_cap;
 }
 
+static size_t wgrad_part_floats(const SelectorState* S, int B) {
+  const size_t splits = (static_cast<size_t>(B) + kWgradK - 1) / kWgradK;
+  return splits * std::max(static_cast<size_t>(kHidden + 1) * S->P, static_cast<size_t>(S->F + 1) * kHidden);
+}
+
 static hipError_t ensure_scratch(SelectorState* S, int B) {
   if (S->scratch_B >= B) return hipSuccess;
   const size_t bytes = sizeof(float) * (static_cast<size_t>(B) * (S->F + 3 * kHidden + 2 * S->P + 1)) +
-                       sizeof(int) * B + 1024;
+                       sizeof(int) * B + 1024 + sizeof(float) * wgrad_part_floats(S, B) + 256;
   hipError_t e = S->scratch.reserve(bytes);
   if (e == hipSuccess) S->scratch_B = B;
   return e;
@@ -747,6 +797,7 @@ static TrainArgs train_args(SelectorState* S, int B) {
   T.loss = p;
   p += B;
   T.idx = reinterpret_cast<int*>(p);
+  T.part = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(T.idx + B + 64) & ~uintptr_t(255));
   T.gW1 = S->gW1;
   T.gb1 = S->gb1;
   T.gW2 = S->gW2;
@@ -760,10 +811,19 @@ static void launch_fwd_bwd(wsbb_ctx* ctx, const TrainArgs& T) {
   hipLaunchKernelGGL(k_tr_in, g(1LL * T.B * T.F), dim3(256), 0, st, T);
   hipLaunchKernelGGL(k_tr_hidden, g(1LL * T.B * kHidden), dim3(256), 0, st, T);
   hipLaunchKernelGGL(k_tr_out, g(1LL * T.B * T.P), dim3(256), 0, st, T);
-  hipLaunchKernelGGL(k_tr_softmax, g(T.B), dim3(256), 0, st, T);
-  hipLaunchKernelGGL(k_tr_gW2, g(1LL * kHidden * T.P + T.P), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_tr_softmax, dim3((T.B + 3) / 4), dim3(256), 0, st, T);
+  const int splits = (T.B + kWgradK - 1) / kWgradK;
+  // gW2 = Hb^T dZ, gb2 = column sums of dZ
+  hipLaunchKernelGGL(k_tr_wgrad, dim3((kHidden + 1 + 63) / 64, (T.P + 63) / 64, splits), dim3(256), 0,
+                     st, T.Hb, kHidden, T.dZ, T.P, T.B, T.part);
+  hipLaunchKernelGGL(k_tr_wreduce, g(1LL * (kHidden + 1) * T.P), dim3(256), 0, st, T.part, kHidden,
+                     T.P, splits, T.gW2, T.gb2);
   hipLaunchKernelGGL(k_tr_dH, g(1LL * T.B * kHidden), dim3(256), 0, st, T);
-  hipLaunchKernelGGL(k_tr_gW1, g(1LL * T.F * kHidden + kHidden), dim3(256), 0, st, T);
+  // gW1 = X^T dH, gb1 = column sums of dH
+  hipLaunchKernelGGL(k_tr_wgrad, dim3((T.F + 1 + 63) / 64, (kHidden + 63) / 64, splits), dim3(256), 0,
+                     st, T.X, T.F, T.dH, kHidden, T.B, T.part);
+  hipLaunchKernelGGL(k_tr_wreduce, g(1LL * (T.F + 1) * kHidden), dim3(256), 0, st, T.part, T.F,
+                     kHidden, splits, T.gW1, T.gb1);
 }
 
 }  // namespace wsbb
