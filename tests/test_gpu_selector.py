@@ -307,3 +307,39 @@ def test_trained_selector_gains_on_20x20(parents):
     assert us["bounded"] > 1.5 * ls["bounded"] or not untrained["proven"]
     print(f"plain {ps['bounded']} nodes {ps['pairs_evaluated']} pairs; learned {ls['bounded']} "
           f"nodes {ls['pairs_evaluated']} pairs; untrained {us['bounded']} nodes")
+
+
+def test_online_training_resumes_to_the_optimum():
+    """Online learning on ta024 (20x20): a plain solve logs records for its
+    first 2e6 bounded children and stops, the search is checkpointed, the
+    network is trained from scratch on those records and the search resumes
+    with learned selection (top-24 pairs per tile, no fallback).  The resumed
+    search proves the optimum plain LB2 proves, with a small node
+    inflation, and evaluates a fraction of the pairs after the switch."""
+    p = oracle.taillard(24)[3]
+    pool, nb, cap = 65536, 4, 1 << 17
+    with wsbb_amd.Engine(p) as eng:
+        plain = eng.solve(pool_size=pool, max_batches=nb)
+        eng.selector_init(seed=1)
+        eng.selector_config(topk=24, margin=-1, parents=True)
+        eng.selector_log(stride=max(1, 2_000_000 * 20 // cap), capacity=cap)
+        first = eng.solve(pool_size=pool, max_batches=nb, budget=2_000_000)
+        assert not first["proven"]
+        ck = eng.checkpoint()
+        feat, label = eng.selector_records()
+        assert 0 < feat.shape[1] <= cap
+        assert ((label >= -1) & (label < eng.P)).all() and (label >= 0).any()
+        eng.selector_set_records(feat, label)
+        losses = eng.selector_train(epochs=2, batch=2048, lr=2e-3, seed=2)
+        assert losses[-1] < losses[0]
+        eng.selector_log(0, 0)
+        res = eng.resume(ck, kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
+    assert plain["proven"] and res["proven"]
+    assert res["best"] == plain["best"]
+    assert oracle.Oracle(p).cmax(res["perm"]) == res["best"]
+    ps, rs, fs = plain["stats"], res["stats"], first["stats"]
+    assert rs["bounded"] < 1.3 * ps["bounded"]
+    after = rs["pairs_evaluated"] - fs["pairs_evaluated"]
+    assert after < 0.3 * 190 * (rs["bounded"] - fs["bounded"])
+    print(f"plain {ps['bounded']} nodes; online {rs['bounded']} nodes, "
+          f"{after / max(rs['bounded'] - fs['bounded'], 1):.1f} pairs per node after the switch")
