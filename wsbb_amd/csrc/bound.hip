@@ -4,9 +4,9 @@
 // pair and the same Johnson step at the same time, so every per-step constant
 // (job, p_a, p_b, lag) is wave-uniform and arrives through scalar loads into
 // SGPRs; per lane a step is one bit extract from the node's unscheduled-job
-// mask, three 24-bit multiply-adds and a max.  The pair/step stream is
-// software-pipelined: the scalar loads of the next group of four steps are in
-// flight while the current group computes, across pair boundaries.  Heads,
+// mask, three 24-bit multiply-adds and a max.  The two-node int16 form
+// (bound_kernel_pk) streams its step table eight steps per scalar load,
+// with the next eight in flight while the current ones compute.  Heads,
 // tails and the job mask live in VGPRs (read with wave-uniform indices); the
 // processing-time matrix and each lane's permutation are staged in LDS for
 // the per-lane gathers of the head/tail recurrences.  `groups` waves may
@@ -393,51 +393,69 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
 // ---- two nodes per lane in int16 halves ------------------------------------
 // Used for plain LB2 (no optional outputs, no `best` exit) when every bound
 // value of the instance stays below 2^14: lane l of a 128-node tile bounds
-// nodes 128 t + l (low halves) and 128 t + 64 + l (high halves).  Heads and
-// tails are built per node as in bound_kernel and packed; a Johnson step is
-// then two bit extracts (plus word selects for n > 32), one pack, three
-// v_pk_mad_u16 and a v_pk_max_i16 for both nodes, with no scalar work.  The
-// arithmetic is exact: every value, including the clock biased by -2^14,
-// fits a signed 16-bit half.
+// nodes 128 t + l (low halves) and 128 t + 64 + l (high halves).  Heads,
+// tails and remaining work are built per node as in bound_kernel and
+// packed; a Johnson step is then two bit extracts (plus word selects for
+// n > 32), one pack, two v_pk_mad_u16 and a v_pk_max_i16 for both nodes
+// (the offset form below).  The arithmetic is exact: every value, including
+// the clock biased by -2^14 and the offsets by at most the remaining work,
+// fits a signed 16-bit half while the path bound stays below 2^14.
+//
+// Tail combine.  After the scan T0 = head(a) + the remaining work of
+// machine a, whatever the pair's Johnson order, so the term T0 + q[a] of
+// cmaxFin is the one-machine bound of machine a; every machine a <= m - 2
+// is the first machine of some pair, so with a suffix LB2 = max(max_a
+// (head(a) + rem(a) + q[a]), max_s (t1_s + q[b_s])), and without one
+// max_s (t1_s + Q0[b_s]) (borneInfMakespan 146-148).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s16x2 as_s16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
 __device__ __forceinline__ uint32_t as_u32(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
 
-// four steps for two nodes: the job's bit of each node's 32-bit mask word
-// (word chosen by the step's select masks, as in the one-node form), packed
-// into the two halves, then three packed multiply-adds and a packed max
+// four steps for two nodes, in offset form.  With S the running sum of p_b
+// over the unscheduled jobs passed so far, the scan keeps u = t1 - S and
+// D = T0 - S; an unscheduled job's step t0 += p_a; t1 = max(t1, t0 + lag)
+// + p_b becomes
+//   D += p_a - p_b;  u = max(u, D + 2^14 + lag + p_b)
+// and a scheduled job (f = 0) leaves D and gives max(u, D) = u, since
+// D <= u whenever T0 <= t1.  At the end S = rem(b), so t1 = u + rem(b).
+// The flags: the job's bit of each node's 32-bit mask word (word chosen by
+// the step's select masks, as in the one-node form), packed into the halves.
 template <int NW>
-__device__ __forceinline__ void step4_pk(const int4 (&rec)[4], const uint2 (&sel)[4],
-                                         const uint32_t (&uA)[NW], const uint32_t (&uB)[NW],
-                                         s16x2& T0, s16x2& t1) {
+__device__ __forceinline__ void step4_pk(const uint2 (&rec)[4], const uint32_t (&uA)[NW],
+                                         const uint32_t (&uB)[NW], s16x2& D, s16x2& u) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int4 qq = rec[u];
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t j = rec[k].x;  // job: bit offset in its low 5 bits, word above
     uint32_t wA, wB;
     if constexpr (NW == 1) {
       wA = uA[0];
       wB = uB[0];
-    } else if constexpr (NW == 2) {
-      wA = bitsel(sel[u].x, uA[1], uA[0]);
-      wB = bitsel(sel[u].x, uB[1], uB[0]);
     } else {
-      wA = bitsel(sel[u].y, bitsel(sel[u].x, uA[3], uA[2]), bitsel(sel[u].x, uA[1], uA[0]));
-      wB = bitsel(sel[u].y, bitsel(sel[u].x, uB[3], uB[2]), bitsel(sel[u].x, uB[1], uB[0]));
+      // word-select masks from the job's bits 5 and 6 (scalar ops)
+      const uint32_t s5 = static_cast<uint32_t>(-static_cast<int>((j >> 5) & 1u));
+      wA = bitsel(s5, uA[1], uA[0]);
+      wB = bitsel(s5, uB[1], uB[0]);
+      if constexpr (NW == 4) {
+        const uint32_t s6 = static_cast<uint32_t>(-static_cast<int>((j >> 6) & 1u));
+        wA = bitsel(s6, bitsel(s5, uA[3], uA[2]), wA);
+        wB = bitsel(s6, bitsel(s5, uB[3], uB[2]), wB);
+      }
     }
-    const uint32_t j = static_cast<uint32_t>(qq.x);  // bit offset in its low 5 bits
     const uint32_t fA = __builtin_amdgcn_ubfe(wA, j, 1u);
     const uint32_t fB = __builtin_amdgcn_ubfe(wB, j, 1u);
     const s16x2 f = as_s16((fB << 16) | fA);
-    T0 = f * as_s16(qq.y) + T0;
-    const s16x2 x = f * as_s16(qq.z) + t1;
-    const s16x2 y = f * as_s16(qq.w) + T0;
-    t1 = __builtin_elementwise_max(x, y);
+    // one word holds both constants: p_a - p_b (low half), 2^14 + lag + p_b
+    // (high half); each multiply-add reads its half in both lanes
+    const s16x2 c = as_s16(rec[k].y);
+    D = f * c.xx + D;
+    const s16x2 z = f * c.yy + D;
+    u = __builtin_elementwise_max(u, z);
   }
 }
 
 template <int MM, int NW, class Src>
-__global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, BoundArgs A) {
+__global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTables T, Src src, BoundArgs A) {
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
   const int mn4 = (MM * n + 3) & ~3;
@@ -495,81 +513,125 @@ __global__ __launch_bounds__(256) void bound_kernel_pk(DevTables T, Src src, Bou
       }
     }
     if (G > 1) __syncthreads();
-    uint32_t rp[MM], qp[MM], uA[NW], uB[NW];
-    uint32_t d2m = 0;
+    // rp: start clocks (heads, or R0 without prefix); ep: what a pair's t1
+    // still needs at its end, rem + (q with a suffix, Q0 without);
+    // l1m: the one-machine terms of machines 0..m-2 (with a suffix)
+    uint32_t rp[MM], ep[MM], uA[NW], uB[NW];
+    uint32_t l1m = 0;
     int leafv[2];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) rp[k] = qp[k] = 0;
+    for (int k = 0; k < MM; ++k) rp[k] = ep[k] = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint8_t* mine = pl + tib * 128 + h * 64 + lane;
       const int d1 = l1[h] + 1, d2 = n - l2[h];
-      int r[MM], q[MM];
+      // c = tail (0 without suffix) + remaining work, one array at a time
+      // live beside the heads: tails first, then the free positions'
+      // processing times added in, then the heads
+      int r[MM], c[MM];
       uint32_t sm[NW];
 #pragma unroll
-      for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
+      for (int k = 0; k < MM; ++k) r[k] = c[k] = 0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) sm[w] = 0;
-      {
-        int jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
-        for (int i = 0; i < d1; ++i) {
-          const int j = jn;
-          if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
-          set_job_bit<NW>(sm, j);
-          int pk[MM];
-          job_row<MM>(sp, j, pk);
-          head_step<MM>(r, pk);
-        }
-        jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
-        for (int i = n - 1; i >= l2[h]; --i) {
-          const int j = jn;
-          if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
-          set_job_bit<NW>(sm, j);
-          int pk[MM];
-          job_row<MM>(sp, j, pk);
-          tail_step<MM>(q, pk);
-        }
+      int jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
+      for (int i = n - 1; i >= l2[h]; --i) {
+        const int j = jn;
+        if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        tail_step<MM>(c, pk);
       }
-      int lv = 0;
+      const int f_hi = valid[h] ? l2[h] : d1;  // padding lanes read no permutation
+      for (int i = d1; i < f_hi; ++i) {
+        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
 #pragma unroll
-      for (int k = 0; k < MM; ++k) lv = max(lv, r[k] + q[k]);
-      leafv[h] = lv;
+        for (int k = 0; k < MM; ++k) c[k] += pk[k];
+      }
+      jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
+      for (int i = 0; i < d1; ++i) {
+        const int j = jn;
+        if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        head_step<MM>(r, pk);
+      }
+      // a leaf has no free position (c = tail): its makespan
+      int lv = 0, l1v = 0;
 #pragma unroll
       for (int k = 0; k < MM; ++k) {
+        lv = max(lv, r[k] + c[k]);
         const int rk = d1 ? r[k] : ld_const(T.R0, k);  // zero past machine m
+        const int ek = c[k] + (d2 ? 0 : ld_const(T.Q0, k));
+        if (k < m - 1) l1v = max(l1v, rk + c[k]);
         rp[k] |= static_cast<uint32_t>(rk) << (16 * h);
-        qp[k] |= static_cast<uint32_t>(q[k]) << (16 * h);
+        ep[k] |= static_cast<uint32_t>(ek) << (16 * h);
       }
+      leafv[h] = lv;
+      if (d2) l1m |= static_cast<uint32_t>(l1v) << (16 * h);
 #pragma unroll
       for (int w = 0; w < NW; ++w) {
         const uint32_t um = ~sm[w] & valid_bits[w];
         if (h == 0) uA[w] = um;
         else uB[w] = um;
       }
-      d2m |= (d2 ? 0xffffu : 0u) << (16 * h);
     }
-    s16x2 lb = as_s16(0u);
+    s16x2 lb = as_s16(grp == 0 ? l1m : 0u);
     const s16x2 bias = as_s16(static_cast<uint32_t>(kBias16) * 0x10001u);
-    for (int s = s_lo; s < s_hi; ++s) {
-      const int2 ab = ld_const2(T.pairs, s);
-      s16x2 T0 = as_s16(rp[ab.x]) - bias;  // machine-a clocks, biased below zero
-      s16x2 t1 = as_s16(rp[ab.y]);
-      const size_t base = static_cast<size_t>(s) * T.n4;
-      for (int k = 0; k < T.n4; k += 4) {
-        int4 rec[4];
-        uint2 sel[4];
+    // the pairs' step groups are contiguous in the table: walk them as one
+    // stream; the read-ahead past the last group stays inside the table's
+    // 16 padding steps
+    const int gpp = T.n4 >> 2;  // step groups per pair
+    const int g_lo = s_lo * gpp, g_hi = s_hi * gpp;
+    if (g_lo < g_hi) {
+      const uint2* gp = T.steps16 + static_cast<size_t>(g_lo) * 4;
+      uint2 recA[4], recB[4];
+      auto load_group = [&](const uint2* at, uint2 (&rc)[4]) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          rec[u] = ld_const4(T.steps16, base + k + u);
-          if constexpr (NW > 1) sel[u] = ld_const2u(T.wsel, base + k + u);
+        for (int q = 0; q < 4; ++q) rc[q] = ld_const2u(at, q);
+      };
+      int s = s_lo, kg = 0;
+      int2 ab = ld_const2(T.pairs, s);
+      s16x2 D = as_s16(rp[ab.x]) - bias;  // machine-a clocks, biased below zero
+      s16x2 u = as_s16(rp[ab.y]);
+      // at the end of a pair: its value, then the next pair's start clocks
+      auto end_group = [&]() {
+        if (++kg == gpp) {
+          lb = __builtin_elementwise_max(lb, u + as_s16(ep[ab.y]));
+          kg = 0;
+          ab = ld_const2(T.pairs, min(++s, s_hi - 1));
+          D = as_s16(rp[ab.x]) - bias;
+          u = as_s16(rp[ab.y]);
         }
-        step4_pk<NW>(rec, sel, uA, uB, T0, t1);
+      };
+      // eight steps (two groups) per scalar load: chunk c + 1 is loaded
+      // while chunk c computes
+      uint2 recC[4], recD[4];
+      load_group(gp, recA);
+      load_group(gp + 4, recB);
+      for (int g = g_lo;;) {
+        load_group(gp + 8, recC);
+        load_group(gp + 12, recD);
+        step4_pk<NW>(recA, uA, uB, D, u);
+        end_group();
+        if (++g == g_hi) break;
+        step4_pk<NW>(recB, uA, uB, D, u);
+        end_group();
+        if (++g == g_hi) break;
+        load_group(gp + 16, recA);
+        load_group(gp + 20, recB);
+        step4_pk<NW>(recC, uA, uB, D, u);
+        end_group();
+        if (++g == g_hi) break;
+        step4_pk<NW>(recD, uA, uB, D, u);
+        end_group();
+        if (++g == g_hi) break;
+        gp += 16;
       }
-      const uint32_t q0b = static_cast<uint32_t>(ld_const(T.Q0, ab.y)) * 0x10001u;
-      const s16x2 c1 = __builtin_elementwise_max(t1 + as_s16(qp[ab.y]), T0 + bias + as_s16(qp[ab.x]));
-      const s16x2 c0 = t1 + as_s16(q0b);
-      const uint32_t v = (as_u32(c1) & d2m) | (as_u32(c0) & ~d2m);
-      lb = __builtin_elementwise_max(lb, as_s16(v));
     }
     if (G > 1) {
       part[threadIdx.x] = as_u32(lb);

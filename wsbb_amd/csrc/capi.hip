@@ -167,10 +167,10 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   // Pack every table into one upload.  steps[s][k] = {job, p_a, p_b,
   // kBias + lag + p_b} in Johnson order, wsel[s][k] the masks that pick the
   // job's 32-bit word of a node's job mask; padding steps are all zero
-  // (no-ops), and 8 more follow the last pair for the scan's prefetch.
+  // (no-ops), and 16 more follow the last pair for the scans' read-ahead.
   const Instance& I = ctx->inst;
   const int P = I.P, n4 = (n + 3) & ~3;
-  const size_t nsteps = static_cast<size_t>(P) * n4 + 8;
+  const size_t nsteps = static_cast<size_t>(P) * n4 + 16;
   std::vector<int4> steps(nsteps, int4{0, 0, 0, 0});
   std::vector<uint2> wsel(nsteps, uint2{0u, 0u});
   for (int s = 0; s < P; ++s)
@@ -184,21 +184,22 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   // two nodes per lane in int16 halves when every bound value (<= the path
   // bound) and every step constant stays below 2^14
   const bool two16 = I.path_bound < kBias16 - 1;
-  std::vector<int4> steps16(two16 ? nsteps : 0, int4{0, 0, 0, 0});
+  std::vector<uint2> steps16(two16 ? nsteps : 0, uint2{0u, 0u});
   if (two16)
     for (int s = 0; s < P; ++s)
       for (int k = 0; k < n; ++k) {
         const size_t at = static_cast<size_t>(s) * n4 + k;
         const int4 w = steps[at];
         const int K = kBias16 + (w.w - kBias);  // 2^14 + lag + p_b
-        steps16[at] = int4{w.x * 0x10001, w.y * 0x10001, w.z * 0x10001, K * 0x10001};
+        const uint32_t dab = static_cast<uint32_t>(w.y - w.z) & 0xffffu;  // p_a - p_b
+        steps16[at] = uint2{static_cast<uint32_t>(w.x), dab | (static_cast<uint32_t>(K) << 16)};
       }
   std::vector<int2> pairs(P);
   for (int s = 0; s < P; ++s) pairs[s] = int2{I.pair_a[s], I.pair_b[s]};
   const size_t o_steps = 0;
   const size_t o_wsel = o_steps + steps.size() * sizeof(int4);
   const size_t o_s16 = o_wsel + wsel.size() * sizeof(uint2);
-  const size_t o_pairs = o_s16 + steps16.size() * sizeof(int4);
+  const size_t o_pairs = o_s16 + steps16.size() * sizeof(uint2);
   const size_t o_p = (o_pairs + pairs.size() * sizeof(int2) + 255) & ~size_t(255);
   const size_t o_R0 = o_p + ((I.p.size() * 4 + 255) & ~size_t(255));
   const size_t o_Q0 = o_R0 + 256;
@@ -208,7 +209,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   std::vector<char> blob(total, 0);
   std::memcpy(blob.data() + o_steps, steps.data(), steps.size() * sizeof(int4));
   std::memcpy(blob.data() + o_wsel, wsel.data(), wsel.size() * sizeof(uint2));
-  if (two16) std::memcpy(blob.data() + o_s16, steps16.data(), steps16.size() * sizeof(int4));
+  if (two16) std::memcpy(blob.data() + o_s16, steps16.data(), steps16.size() * sizeof(uint2));
   std::memcpy(blob.data() + o_pairs, pairs.data(), pairs.size() * sizeof(int2));
   std::memcpy(blob.data() + o_p, I.p.data(), I.p.size() * 4);
   std::memcpy(blob.data() + o_R0, I.R0.data(), m * 4);
@@ -228,7 +229,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   T.R0 = ctx->tables.as<const int32_t>(o_R0);
   T.Q0 = ctx->tables.as<const int32_t>(o_Q0);
   T.ptot = ctx->tables.as<const int32_t>(o_tot);
-  T.steps16 = two16 ? ctx->tables.as<const int4>(o_s16) : nullptr;
+  T.steps16 = two16 ? ctx->tables.as<const uint2>(o_s16) : nullptr;
   *out = ctx;
   return WSBB_OK;
 }

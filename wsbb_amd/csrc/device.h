@@ -35,15 +35,15 @@ struct DevTables {
   const int32_t* p;      // [m][n]
   const int2* pairs;     // [P] (machine a, machine b)
   const int4* steps;     // [P][n4] {job, p_a, p_b, kBias + lag + p_b}; pad = 0,
-                         // plus 8 zero steps after the last pair
+                         // plus 16 zero steps after the last pair
   const uint2* wsel;     // [P][n4] masks selecting the job's 32-bit mask word:
                          // x = all ones iff bit 0 of (job >> 5), y = bit 1
   const int32_t* R0;     // [m]
   const int32_t* Q0;     // [m]
   const int32_t* ptot;   // [m]
-  const int4* steps16;   // [P][n4] two-node form {job * 0x10001, p_a * 0x10001,
-                         // p_b * 0x10001, (2^14 + lag + p_b) * 0x10001}, or
-                         // null when bound values may reach 2^14
+  const uint2* steps16;  // [P][n4] two-node form {job, (p_a - p_b mod 2^16)
+                         // | (2^14 + lag + p_b) << 16}; pad = 0, plus 16
+                         // zero steps; null when bound values may reach 2^14
 };
 
 // What a bound launch computes and where it writes.
