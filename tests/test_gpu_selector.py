@@ -203,15 +203,19 @@ def test_parent_selector_logging_and_solves():
           f"{nofb['stats']['bounded']}")
 
 
-@pytest.mark.parametrize("tid", [11, 51])
-def test_tile_subset_bounds_match_oracle(tid):
+@pytest.mark.parametrize("tid,N,int16", [(11, 640, 0), (51, 640, 0), (11, 566, 2), (21, 566, 2),
+                                         (51, 640, 2)])
+def test_tile_subset_bounds_match_oracle(tid, N, int16):
+    """Subset bounds per 64-node tile against the oracle, with the two-node
+    form allowed (int16=2: it stays with plain LB2) and without; N = 566
+    leaves a partial last tile."""
     p = oracle.taillard(tid)[3]
     m, n = p.shape
     o = oracle.Oracle(p)
-    N = 640
     permT, l1, l2 = oracle.random_nodes(n, N, 11)
     rng = np.random.default_rng(12)
     with wsbb_amd.Engine(p) as eng:
+        eng.set_int16(int16)
         P, nsw = eng.P, (eng.P + 31) // 32
         selb = rng.random(((N + 63) // 64, P)) < 0.15
         selb[:, 0] = True
