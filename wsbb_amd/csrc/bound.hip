@@ -518,13 +518,20 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
     // l1m: the one-machine terms of machines 0..m-2 (with a suffix)
     uint32_t rp[MM], ep[MM], uA[NW], uB[NW];
     uint32_t l1m = 0;
-    int leafv[2];
+    int leafv0 = 0, leafv1 = 0;
 #pragma unroll
     for (int k = 0; k < MM; ++k) rp[k] = ep[k] = 0;
-#pragma unroll
+    // one node at a time: its head, tail and work arrays are not live
+    // beside the other node's
+#pragma unroll 1
     for (int h = 0; h < 2; ++h) {
+      // this node's fields by selects (indexing the pairs by h would put
+      // them in scratch memory)
+      const int l1h = h ? l1[1] : l1[0], l2h = h ? l2[1] : l2[0];
+      const int sw0h = h ? sw0[1] : sw0[0], sw1h = h ? sw1[1] : sw1[0];
+      const bool validh = h ? valid[1] : valid[0];
       const uint8_t* mine = pl + tib * 128 + h * 64 + lane;
-      const int d1 = l1[h] + 1, d2 = n - l2[h];
+      const int d1 = l1h + 1, d2 = n - l2h;
       // c = tail (0 without suffix) + remaining work, one array at a time
       // live beside the heads: tails first, then the free positions'
       // processing times added in, then the heads
@@ -534,27 +541,27 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
       for (int k = 0; k < MM; ++k) r[k] = c[k] = 0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) sm[w] = 0;
-      int jn = d2 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], n - 1) : 0;
-      for (int i = n - 1; i >= l2[h]; --i) {
+      int jn = d2 > 0 ? perm_at(mine, bst, sw0h, sw1h, n - 1) : 0;
+      for (int i = n - 1; i >= l2h; --i) {
         const int j = jn;
-        if (i - 1 >= l2[h]) jn = perm_at(mine, bst, sw0[h], sw1[h], i - 1);
+        if (i - 1 >= l2h) jn = perm_at(mine, bst, sw0h, sw1h, i - 1);
         set_job_bit<NW>(sm, j);
         int pk[MM];
         job_row<MM>(sp, j, pk);
         tail_step<MM>(c, pk);
       }
-      const int f_hi = valid[h] ? l2[h] : d1;  // padding lanes read no permutation
+      const int f_hi = validh ? l2h : d1;  // padding lanes read no permutation
       for (int i = d1; i < f_hi; ++i) {
-        const int j = perm_at(mine, bst, sw0[h], sw1[h], i);
+        const int j = perm_at(mine, bst, sw0h, sw1h, i);
         int pk[MM];
         job_row<MM>(sp, j, pk);
 #pragma unroll
         for (int k = 0; k < MM; ++k) c[k] += pk[k];
       }
-      jn = d1 > 0 ? perm_at(mine, bst, sw0[h], sw1[h], 0) : 0;
+      jn = d1 > 0 ? perm_at(mine, bst, sw0h, sw1h, 0) : 0;
       for (int i = 0; i < d1; ++i) {
         const int j = jn;
-        if (i + 1 < d1) jn = perm_at(mine, bst, sw0[h], sw1[h], i + 1);
+        if (i + 1 < d1) jn = perm_at(mine, bst, sw0h, sw1h, i + 1);
         set_job_bit<NW>(sm, j);
         int pk[MM];
         job_row<MM>(sp, j, pk);
@@ -571,7 +578,8 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
         rp[k] |= static_cast<uint32_t>(rk) << (16 * h);
         ep[k] |= static_cast<uint32_t>(ek) << (16 * h);
       }
-      leafv[h] = lv;
+      if (h) leafv1 = lv;
+      else leafv0 = lv;
       if (d2) l1m |= static_cast<uint32_t>(l1v) << (16 * h);
 #pragma unroll
       for (int w = 0; w < NW; ++w) {
@@ -645,7 +653,8 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
 #pragma unroll
       for (int h = 0; h < 2; ++h)
         if (valid[h]) {
-          const int res = (l2[h] - l1[h] == 1) ? leafv[h] : static_cast<int>((lbu >> (16 * h)) & 0xffffu);
+          const int res = (l2[h] - l1[h] == 1) ? (h ? leafv1 : leafv0)
+                                               : static_cast<int>((lbu >> (16 * h)) & 0xffffu);
           A.out[src.slot(node[h])] = res;
         }
     }
