@@ -96,11 +96,13 @@ template <int PB, int OUT>
 __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   extern __shared__ uint2 s_w[];
   // the grid is sized for the largest batch: blocks without work leave
-  // before staging the weights
+  // before staging the weights.  Work items go wave-major over the blocks
+  // (item = wave * grid + block), so a small batch spreads over every CU
+  // instead of filling the first blocks' waves
   {
     const int N0 = A.Ndev ? *A.Ndev : A.N;
     const int nw0 = A.tile_list ? A.nlist : OUT == 2 ? (N0 + 31) >> 5 : (N0 + 63) >> 6;
-    if (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x >> 6) >= nw0) return;
+    if (static_cast<int>(blockIdx.x) >= nw0) return;
   }
   const int n1 = 4 * A.fsteps * 64, n2 = PB * 16 * 64;
   // 16 loads in flight per thread before their LDS stores: the layer-2
@@ -133,9 +135,8 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
   const int col = lane & 31, half = lane >> 5;
   // probabilities only: a work item is one 32-node block
   const int nwork = A.tile_list ? A.nlist : OUT == 2 ? (N + 31) >> 5 : ntiles;
-  for (int w0 = blockIdx.x * nwaves; w0 < nwork; w0 += gridDim.x * nwaves) {
-    const int wi = __builtin_amdgcn_readfirstlane(w0 + wave);
-    if (wi >= nwork) continue;  // no block barrier below
+  for (int w0 = wave * gridDim.x + blockIdx.x; w0 < nwork; w0 += gridDim.x * nwaves) {
+    const int wi = __builtin_amdgcn_readfirstlane(w0);  // no block barrier below
     const int tile = A.tile_list ? __builtin_amdgcn_readfirstlane(A.tile_list[wi]) : wi;
     const int tcnt = A.tile_cnt ? A.tile_cnt[tile] : 64;
     float agg[PB];  // tile sums of the probabilities of pairs 32 bp + col
