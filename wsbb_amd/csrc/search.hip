@@ -435,15 +435,22 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
 // two jobs each child places instead of rerunning the recurrences over the
 // whole prefix and suffix per child.  Written for latency (a round has a few
 // thousand parents): one batch of loads per column chunk, no lane exchanges.
+// small blocks: a round's parents (a few thousand) spread over many CUs
+// instead of filling a few
+constexpr int kExpandBlock = 64;
+
 template <int MM, int NW, bool FEAT>
-__global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_state) {
+__global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, int want_state) {
   extern __shared__ int32_t lds[];
   int32_t* sp = lds;                                          // [n][MM] job-major
-  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * T.n);    // [n][256] own columns
-  stage_times<MM>(T, sp);
-  __syncthreads();
+  uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * T.n);    // [n][block] own columns
   const Ctrl* C = S.ctrl;
   const int np = C->n_parents;
+  // the grid covers the parent capacity: blocks past the round's parents
+  // leave before staging
+  if (static_cast<int>(blockIdx.x * blockDim.x) >= np) return;
+  stage_times<MM>(T, sp);
+  __syncthreads();
   const int n = S.n, m = S.m, P = S.P;
   const int lane = threadIdx.x & 63;
   const size_t PH = static_cast<size_t>(S.PCAP);  // rows of the parent state, as ChildSource reads them
@@ -462,7 +469,7 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
       for (int u = 0; u < 32; ++u)
         if (i0 + u < n) {
           S.p_perm[static_cast<size_t>(i0 + u) * S.PCAP + p] = static_cast<uint8_t>(v[u]);
-          pl[(i0 + u) * 256 + threadIdx.x] = static_cast<uint8_t>(v[u]);
+          pl[(i0 + u) * kExpandBlock + threadIdx.x] = static_cast<uint8_t>(v[u]);
         }
     }
     S.p_l1[p] = l1;
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
     int jn = l1 >= 0 ? mine[0] : 0;
     for (int i = 0; i <= l1; ++i) {
       const int j = jn;
-      if (i + 1 <= l1) jn = mine[(i + 1) * 256];
+      if (i + 1 <= l1) jn = mine[(i + 1) * kExpandBlock];
       set_job_bit<NW>(sm, j);
       int pk[MM];
       job_row<MM>(sp, j, pk);
@@ -504,10 +511,10 @@ __global__ __launch_bounds__(256) void k_expand(SArgs S, DevTables T, int want_s
 #pragma unroll
       for (int k = 0; k < MM; ++k) wk[k] += pk[k];
     }
-    jn = l2 < n ? mine[(n - 1) * 256] : 0;
+    jn = l2 < n ? mine[(n - 1) * kExpandBlock] : 0;
     for (int i = n - 1; i >= l2; --i) {
       const int j = jn;
-      if (i - 1 >= l2) jn = mine[(i - 1) * 256];
+      if (i - 1 >= l2) jn = mine[(i - 1) * kExpandBlock];
       set_job_bit<NW>(sm, j);
       int pk[MM];
       job_row<MM>(sp, j, pk);
@@ -568,8 +575,8 @@ hipError_t launch_expand(const wsbb_ctx* ctx, const SArgs& A, int want_state, hi
   const int m = ctx->T.m, n = ctx->T.n;
   const int mm = m <= 8 ? 8 : (m <= 12 ? 12 : (m <= 16 ? 16 : (m <= 20 ? 20 : 24)));
   const int nw = n <= 32 ? 1 : (n <= 64 ? 2 : 4);
-  const dim3 grid((A.PCAP + 255) / 256), block(256);
-  const size_t shmem = sizeof(int32_t) * mm * n + static_cast<size_t>(n) * 256;
+  const dim3 grid((A.PCAP + kExpandBlock - 1) / kExpandBlock), block(kExpandBlock);
+  const size_t shmem = sizeof(int32_t) * mm * n + static_cast<size_t>(n) * kExpandBlock;
   // feature output only where the constant-index form applies (m == MM)
   const bool feat = (A.x_frag != nullptr || A.x_f32 != nullptr);
   if (feat && m != mm) return hipErrorInvalidValue;
