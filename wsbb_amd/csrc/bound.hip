@@ -153,14 +153,18 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     if constexpr (Src::kState && !EXTRA) from_parent = src.has_state();
     const uint8_t* mine = pl + tib * 64 + lane;
     const int bstride = 64 * tpb;
+    // (sources with a parent state stage only when it is absent: smaller
+    // chunks keep that rarely used path from setting the kernel's register
+    // count, which decides how many waves fit per SIMD)
+    constexpr int kChunk = Src::kState ? 4 : 16;
     if (!from_parent && grp == 0 && valid) {
-      for (int i0 = 0; i0 < n; i0 += 16) {
-        uint32_t v[16];
+      for (int i0 = 0; i0 < n; i0 += kChunk) {
+        uint32_t v[kChunk];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < kChunk; ++u)
           if (i0 + u < n) v[u] = col[static_cast<size_t>(i0 + u) * stride];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < kChunk; ++u)
           if (i0 + u < n) pl[(i0 + u) * bstride + tib * 64 + lane] = static_cast<uint8_t>(v[u]);
       }
     }
@@ -174,15 +178,20 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     // boundary of the reversed recurrence.
     int r[MM], q[MM];
     uint32_t sm[NW];
-#pragma unroll
-    for (int k = 0; k < MM; ++k) r[k] = q[k] = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) sm[w] = 0;
-
     if (from_parent) {
-      if constexpr (Src::kState)
-        if (valid) src.template state<MM, NW>(node, sp, r, q, sm);
+      // every lane reads a child's state (padding lanes the tile's last
+      // child, their results unused), so the arrays are assigned on this
+      // path without a zero fill held in registers beside them
+      if constexpr (Src::kState) src.template state<MM, NW>(min(node, N - 1), sp, r, q, sm);
     } else {
+      // a zero the compiler cannot hoist: hoisted, the fill below became a
+      // block of registers held across the tile loop
+      int zero;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+#pragma unroll
+      for (int k = 0; k < MM; ++k) r[k] = q[k] = zero;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sm[w] = 0;
       // the next job index is read while the current row is processed
       int jn = d1 > 0 ? perm_at(mine, bstride, sw0, sw1, 0) : 0;
       for (int i = 0; i < d1; ++i) {

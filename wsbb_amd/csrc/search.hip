@@ -355,10 +355,14 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   __shared__ int s_state[4];
   Ctrl* C = S.ctrl;
   const int tid = threadIdx.x;
+  // every control field in one round of loads; the counter updates at the
+  // end are atomics whose results are not waited on
+  const int status = C->status, top = C->stack_size, inc = C->inc;
+  const long long decomposed0 = C->stats[kStDecomposed];
   // a round after the search ended (or with nothing to pop) does nothing;
   // the fallback count is cleared with the others so that the statistics
   // kernel of such a round adds zero
-  if (C->status != kRunning) {
+  if (status != kRunning) {
     if (tid == 0) {
       C->n_parents = 0;
       C->n_children = 0;
@@ -366,7 +370,6 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     }
     return;
   }
-  const int top = C->stack_size;
   if (top == 0) {
     if (tid == 0) {
       C->status = kDone;
@@ -376,7 +379,6 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     }
     return;
   }
-  const int inc = C->inc;
   // block-uniform carried state: reservation of the open sub-batch, closed
   // sub-batches, parents taken, children, pruned candidates, consumed; a
   // small first window for small pools (most of their rounds end in it),
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
       C->n_children = 0;
       return;
     }
-    if (S.budget > 0 && np > 0 && C->stats[kStDecomposed] + np > S.budget) {
+    if (S.budget > 0 && np > 0 && decomposed0 + np > S.budget) {
       // stop before this round: the stack keeps every candidate, so a
       // resumed search continues exactly where this one stopped
       C->status = kBudget;
@@ -405,9 +407,13 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
       C->n_children = 0;
       return;
     }
-    C->stats[kStRounds] += 1;
+    auto add = [&](int k, long long v) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[k]),
+                static_cast<unsigned long long>(v));
+    };
+    add(kStRounds, 1);
     S.p_off[np] = nc;
-    C->stats[kStPrunedPop] += pruned;
+    if (pruned) add(kStPrunedPop, pruned);
     C->stack_size = top - consumed;
     if (np == 0) {
       C->status = kDone;
@@ -416,12 +422,12 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     } else {
       C->n_parents = np;
       C->n_children = nc;
-      C->stats[kStDecomposed] += np;
-      C->stats[kStBounded] += nc;
-      if (S.count_pairs == 1) C->stats[kStPairs] += static_cast<long long>(nc) * S.P;
+      add(kStDecomposed, np);
+      add(kStBounded, nc);
+      if (S.count_pairs == 1) add(kStPairs, static_cast<long long>(nc) * S.P);
       if (S.count_pairs == 2) {
-        C->stats[kStPairs] += static_cast<long long>(nc) * S.topk;
-        C->stats[kStSubset] += nc;
+        add(kStPairs, static_cast<long long>(nc) * S.topk);
+        add(kStSubset, nc);
       }
     }
   }
@@ -757,6 +763,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   if (np == 0) return;
   const int n = S.n, tid = threadIdx.x;
   const int inc0 = C->inc;
+  const int sbase = C->stack_size;  // read with the other control fields
   if (tid == 0) s_last = -1;
   int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
   int improved = 0, last = -1;
@@ -850,14 +857,14 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   const int bl = s_last;
   if (tid == 0) {
     if (bl >= 0) C->inc = run_min;
-    const int sbase = C->stack_size;
     C->push_base = sbase;
     C->push_total = run_kept;
     if (static_cast<long long>(sbase) + run_kept > S.CAP) {
       C->status = kOverflow;
     } else {
       C->stack_size = sbase + run_kept;
-      if (sbase + run_kept > C->stats[kStMaxPool]) C->stats[kStMaxPool] = sbase + run_kept;
+      atomicMax(reinterpret_cast<unsigned long long*>(&C->stats[kStMaxPool]),
+                static_cast<unsigned long long>(sbase + run_kept));
     }
   }
   // the leaf that set the incumbent: parent permutation with its swap
@@ -876,10 +883,11 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
 // ---- push survivors -------------------------------------------------------
 __global__ __launch_bounds__(256) void k_push(SArgs S) {
   const Ctrl* C = S.ctrl;
-  if (C->status == kOverflow) return;
-  const int nc = C->n_children;
-  const int n = S.n;
+  // every control field in one round of loads, then the overflow test
+  const int status = C->status, nc0 = C->n_children;
   const int base = C->push_base, total = C->push_total;
+  const int nc = status == kOverflow ? 0 : nc0;
+  const int n = S.n;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     // the child's and its parent's fields load in two rounds, whatever the
     // child's fate
