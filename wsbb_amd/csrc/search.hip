@@ -1192,7 +1192,9 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   // more waves than a full batch of CCAP would need (measured on 20x10:
   // four waves per tile beat one, two, eight and sixteen)
   {
-    const int tiles = (A.CCAP + 63) / 64, simds = 4 * ctx->num_cus;
+    // in the kernel's own tiles (128 nodes in the two-node form)
+    const int tiles = pk_round ? (A.CCAP + 127) / 128 : (A.CCAP + 63) / 64;
+    const int simds = 4 * ctx->num_cus;
     B.groups = ctx->launch.groups > 1 ? ctx->launch.groups
                : tiles < 2 * simds    ? 4
                : tiles < 4 * simds    ? 2
