@@ -403,8 +403,8 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
 // Used for plain LB2 (no optional outputs, no `best` exit) when every bound
 // value of the instance stays below 2^14: lane l of a 128-node tile bounds
 // nodes 128 t + l (low halves) and 128 t + 64 + l (high halves).  Heads,
-// tails and remaining work are built per node as in bound_kernel and
-// packed; a Johnson step is then two bit extracts (plus word selects for
+// tails and remaining work are built for both nodes at once in the int16
+// halves; a Johnson step is then two bit extracts (plus word selects for
 // n > 32), one pack, two v_pk_mad_u16 and a v_pk_max_i16 for both nodes
 // (the offset form below).  The arithmetic is exact: every value, including
 // the clock biased by -2^14 and the offsets by at most the remaining work,
@@ -467,11 +467,12 @@ template <int MM, int NW, class Src>
 __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTables T, Src src, BoundArgs A) {
   extern __shared__ int32_t lds[];
   const int n = T.n, m = T.m, P = T.P;
-  const int mn4 = (MM * n + 3) & ~3;
-  int32_t* sp = lds;                                           // [n][MM]
+  const int mn4 = (MM * (n + 1) + 3) & ~3;
+  int32_t* sp = lds;                                           // [n + 1][MM], row n zero
   uint32_t* part = reinterpret_cast<uint32_t*>(lds + mn4);     // [blockDim]
   uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);  // [n][128 tpb]
   stage_times<MM>(T, sp);
+  for (int i = threadIdx.x; i < MM; i += blockDim.x) sp[n * MM + i] = 0;
   __syncthreads();
   const int G = A.groups;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -525,77 +526,94 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
     // rp: start clocks (heads, or R0 without prefix); ep: what a pair's t1
     // still needs at its end, rem + (q with a suffix, Q0 without);
     // l1m: the one-machine terms of machines 0..m-2 (with a suffix)
+    // Heads, tails and remaining work of both nodes at once, one int16 half
+    // each.  At a position only one node uses, the other half reads the
+    // zero row n: a zero step leaves heads (non-decreasing over machines)
+    // and tails (non-increasing) unchanged.
     uint32_t rp[MM], ep[MM], uA[NW], uB[NW];
-    uint32_t l1m = 0;
-    int leafv0 = 0, leafv1 = 0;
+    uint32_t rP[MM], cP[MM];  // heads; tails (0 without suffix) + work
+    uint32_t smA[NW], smB[NW];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) rp[k] = ep[k] = 0;
-    // one node at a time: its head, tail and work arrays are not live
-    // beside the other node's
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
-      // this node's fields by selects (indexing the pairs by h would put
-      // them in scratch memory)
-      const int l1h = h ? l1[1] : l1[0], l2h = h ? l2[1] : l2[0];
-      const int sw0h = h ? sw0[1] : sw0[0], sw1h = h ? sw1[1] : sw1[0];
-      const bool validh = h ? valid[1] : valid[0];
-      const uint8_t* mine = pl + tib * 128 + h * 64 + lane;
-      const int d1 = l1h + 1, d2 = n - l2h;
-      // c = tail (0 without suffix) + remaining work, one array at a time
-      // live beside the heads: tails first, then the free positions'
-      // processing times added in, then the heads
-      int r[MM], c[MM];
-      uint32_t sm[NW];
+    for (int k = 0; k < MM; ++k) rP[k] = cP[k] = 0;
 #pragma unroll
-      for (int k = 0; k < MM; ++k) r[k] = c[k] = 0;
+    for (int w = 0; w < NW; ++w) smA[w] = smB[w] = 0;
+    const uint8_t* mA = pl + tib * 128 + lane;
+    const uint8_t* mB = mA + 64;
+    const int d1A = l1[0] + 1, d1B = l1[1] + 1;
+    // both halves' job rows at positions iA, iB packed (job n: zero row)
+    auto rows = [&](int jA, int jB, uint32_t (&pP)[MM]) {
+      const int4* ra = reinterpret_cast<const int4*>(sp + jA * MM);
+      const int4* rb = reinterpret_cast<const int4*>(sp + jB * MM);
 #pragma unroll
-      for (int w = 0; w < NW; ++w) sm[w] = 0;
-      int jn = d2 > 0 ? perm_at(mine, bst, sw0h, sw1h, n - 1) : 0;
-      for (int i = n - 1; i >= l2h; --i) {
-        const int j = jn;
-        if (i - 1 >= l2h) jn = perm_at(mine, bst, sw0h, sw1h, i - 1);
-        set_job_bit<NW>(sm, j);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-        tail_step<MM>(c, pk);
+      for (int k4 = 0; k4 < MM / 4; ++k4) {
+        const int4 a = ra[k4], b = rb[k4];
+        pP[4 * k4] = static_cast<uint32_t>(a.x) | (static_cast<uint32_t>(b.x) << 16);
+        pP[4 * k4 + 1] = static_cast<uint32_t>(a.y) | (static_cast<uint32_t>(b.y) << 16);
+        pP[4 * k4 + 2] = static_cast<uint32_t>(a.z) | (static_cast<uint32_t>(b.z) << 16);
+        pP[4 * k4 + 3] = static_cast<uint32_t>(a.w) | (static_cast<uint32_t>(b.w) << 16);
       }
-      const int f_hi = validh ? l2h : d1;  // padding lanes read no permutation
-      for (int i = d1; i < f_hi; ++i) {
-        const int j = perm_at(mine, bst, sw0h, sw1h, i);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
+    };
+    // tails over the suffixes
+    for (int i = n - 1; i >= min(l2[0], l2[1]); --i) {
+      const bool aA = i >= l2[0], aB = i >= l2[1];
+      const int pa = perm_at(mA, bst, sw0[0], sw1[0], i), pb = perm_at(mB, bst, sw0[1], sw1[1], i);
+      if (aA) set_job_bit<NW>(smA, pa);
+      if (aB) set_job_bit<NW>(smB, pb);
+      uint32_t pP[MM];
+      rows(aA ? pa : n, aB ? pb : n, pP);
 #pragma unroll
-        for (int k = 0; k < MM; ++k) c[k] += pk[k];
+      for (int k = MM - 1; k >= 0; --k) {
+        const s16x2 nx = as_s16(k + 1 < MM ? cP[k + 1] : 0u);
+        cP[k] = as_u32(__builtin_elementwise_max(as_s16(cP[k]), nx) + as_s16(pP[k]));
       }
-      jn = d1 > 0 ? perm_at(mine, bst, sw0h, sw1h, 0) : 0;
-      for (int i = 0; i < d1; ++i) {
-        const int j = jn;
-        if (i + 1 < d1) jn = perm_at(mine, bst, sw0h, sw1h, i + 1);
-        set_job_bit<NW>(sm, j);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-        head_step<MM>(r, pk);
+    }
+    // remaining work: the free positions (padding lanes read none)
+    {
+      const int fA = valid[0] ? l2[0] : d1A, fB = valid[1] ? l2[1] : d1B;
+      for (int i = min(d1A, d1B); i < max(fA, fB); ++i) {
+        const bool aA = i >= d1A && i < fA, aB = i >= d1B && i < fB;
+        const int pa = perm_at(mA, bst, sw0[0], sw1[0], i), pb = perm_at(mB, bst, sw0[1], sw1[1], i);
+        uint32_t pP[MM];
+        rows(aA ? pa : n, aB ? pb : n, pP);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) cP[k] = as_u32(as_s16(cP[k]) + as_s16(pP[k]));
       }
+    }
+    // heads over the prefixes
+    for (int i = 0; i < max(d1A, d1B); ++i) {
+      const bool aA = i < d1A, aB = i < d1B;
+      const int pa = perm_at(mA, bst, sw0[0], sw1[0], i), pb = perm_at(mB, bst, sw0[1], sw1[1], i);
+      if (aA) set_job_bit<NW>(smA, pa);
+      if (aB) set_job_bit<NW>(smB, pb);
+      uint32_t pP[MM];
+      rows(aA ? pa : n, aB ? pb : n, pP);
+      rP[0] = as_u32(as_s16(rP[0]) + as_s16(pP[0]));
+#pragma unroll
+      for (int k = 1; k < MM; ++k)
+        rP[k] = as_u32(__builtin_elementwise_max(as_s16(rP[k]), as_s16(rP[k - 1])) + as_s16(pP[k]));
+    }
+    // per-half masks: no prefix (R0 instead of heads), a suffix (tails and
+    // one-machine terms count)
+    const uint32_t nd1 = (d1A == 0 ? 0xffffu : 0u) | (d1B == 0 ? 0xffff0000u : 0u);
+    const uint32_t d2m = (l2[0] < n ? 0xffffu : 0u) | (l2[1] < n ? 0xffff0000u : 0u);
+    s16x2 lvP = as_s16(0u), l1P = as_s16(0u);
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
       // a leaf has no free position (c = tail): its makespan
-      int lv = 0, l1v = 0;
+      lvP = __builtin_elementwise_max(lvP, as_s16(rP[k]) + as_s16(cP[k]));
+      const uint32_t r0 = static_cast<uint32_t>(ld_const(T.R0, k)) * 0x10001u;  // zero past m
+      const uint32_t q0 = static_cast<uint32_t>(ld_const(T.Q0, k)) * 0x10001u;
+      rp[k] = (r0 & nd1) | (rP[k] & ~nd1);
+      ep[k] = as_u32(as_s16(cP[k]) + as_s16(q0 & ~d2m));
+      if (k < m - 1) l1P = __builtin_elementwise_max(l1P, as_s16(rp[k]) + as_s16(cP[k]));
+    }
+    const uint32_t l1m = as_u32(l1P) & d2m;
+    const int leafv0 = static_cast<int>(as_u32(lvP) & 0xffffu);
+    const int leafv1 = static_cast<int>(as_u32(lvP) >> 16);
 #pragma unroll
-      for (int k = 0; k < MM; ++k) {
-        lv = max(lv, r[k] + c[k]);
-        const int rk = d1 ? r[k] : ld_const(T.R0, k);  // zero past machine m
-        const int ek = c[k] + (d2 ? 0 : ld_const(T.Q0, k));
-        if (k < m - 1) l1v = max(l1v, rk + c[k]);
-        rp[k] |= static_cast<uint32_t>(rk) << (16 * h);
-        ep[k] |= static_cast<uint32_t>(ek) << (16 * h);
-      }
-      if (h) leafv1 = lv;
-      else leafv0 = lv;
-      if (d2) l1m |= static_cast<uint32_t>(l1v) << (16 * h);
-#pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        const uint32_t um = ~sm[w] & valid_bits[w];
-        if (h == 0) uA[w] = um;
-        else uB[w] = um;
-      }
+    for (int w = 0; w < NW; ++w) {
+      uA[w] = ~smA[w] & valid_bits[w];
+      uB[w] = ~smB[w] & valid_bits[w];
     }
     s16x2 lb = as_s16(grp == 0 ? l1m : 0u);
     const s16x2 bias = as_s16(static_cast<uint32_t>(kBias16) * 0x10001u);
@@ -679,7 +697,7 @@ hipError_t launch_pk(const DevTables& T, const Src& src, const BoundArgs& A, int
   int grid = (ntiles + tpb - 1) / tpb;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + block) +
+  const size_t shmem = sizeof(int32_t) * (((MM * (T.n + 1) + 3) & ~3) + block) +
                        static_cast<size_t>(T.n) * 128 * tpb;
   hipExtLaunchKernelGGL((bound_kernel_pk<MM, NW, Src>), dim3(grid), dim3(block), shmem, stream,
                         A.ev_start, A.ev_stop, 0, T, src, A);
