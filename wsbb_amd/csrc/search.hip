@@ -455,15 +455,21 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, i
   // the grid covers the parent capacity: blocks past the round's parents
   // leave before staging
   if (static_cast<int>(blockIdx.x * blockDim.x) >= np) return;
+  // the grid covers the parent capacity, so a thread has at most one
+  // parent: its stack slot and first child load before the times are
+  // staged, and the two memory round trips overlap
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool mine_p = p < np;
+  const int si = mine_p ? S.p_sel[p] : 0;
+  const int off = mine_p ? S.p_off[p] : 0;
   stage_times<MM>(T, sp);
   __syncthreads();
   const int n = S.n, m = S.m, P = S.P;
   const int lane = threadIdx.x & 63;
   const size_t PH = static_cast<size_t>(S.PCAP);  // rows of the parent state, as ChildSource reads them
   long long work = 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-    const int si = S.p_sel[p];
-    const int l1 = S.s_l1[si], l2 = S.s_l2[si], cost = S.s_cost[si], off = S.p_off[p];
+  if (mine_p) {
+    const int l1 = S.s_l1[si], l2 = S.s_l2[si], cost = S.s_cost[si];
     // loads of a chunk all issue before its stores (the arrays may alias as
     // far as the compiler knows)
     for (int i0 = 0; i0 < n; i0 += 32) {
@@ -496,79 +502,80 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, i
       const long long base = P * (n + 4 * (u - 1LL) + 4);
       work += u * (base + 2LL * m * (d1 + 1) + mm1 * d2) + u * (base + 2LL * m * d1 + mm1 * (d2 + 1));
     }
-    if (!want_state) continue;
-    // the parent's heads and tails (no R0/Q0 substitution: that depends on
-    // the child's own prefix and suffix), scheduled jobs and their work
-    int r[MM], q[MM], wk[MM];
-    uint32_t sm[NW];
+    if (want_state) {
+      // the parent's heads and tails (no R0/Q0 substitution: that depends on
+      // the child's own prefix and suffix), scheduled jobs and their work
+      int r[MM], q[MM], wk[MM];
+      uint32_t sm[NW];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) r[k] = q[k] = wk[k] = 0;
+      for (int k = 0; k < MM; ++k) r[k] = q[k] = wk[k] = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) sm[w] = 0;
-    const uint8_t* mine = pl + threadIdx.x;
-    int jn = l1 >= 0 ? mine[0] : 0;
-    for (int i = 0; i <= l1; ++i) {
-      const int j = jn;
-      if (i + 1 <= l1) jn = mine[(i + 1) * kExpandBlock];
-      set_job_bit<NW>(sm, j);
-      int pk[MM];
-      job_row<MM>(sp, j, pk);
-      head_step<MM>(r, pk);
+      for (int w = 0; w < NW; ++w) sm[w] = 0;
+      const uint8_t* mine = pl + threadIdx.x;
+      int jn = l1 >= 0 ? mine[0] : 0;
+      for (int i = 0; i <= l1; ++i) {
+        const int j = jn;
+        if (i + 1 <= l1) jn = mine[(i + 1) * kExpandBlock];
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        head_step<MM>(r, pk);
 #pragma unroll
-      for (int k = 0; k < MM; ++k) wk[k] += pk[k];
-    }
-    jn = l2 < n ? mine[(n - 1) * kExpandBlock] : 0;
-    for (int i = n - 1; i >= l2; --i) {
-      const int j = jn;
-      if (i - 1 >= l2) jn = mine[(i - 1) * kExpandBlock];
-      set_job_bit<NW>(sm, j);
-      int pk[MM];
-      job_row<MM>(sp, j, pk);
-      tail_step<MM>(q, pk);
+        for (int k = 0; k < MM; ++k) wk[k] += pk[k];
+      }
+      jn = l2 < n ? mine[(n - 1) * kExpandBlock] : 0;
+      for (int i = n - 1; i >= l2; --i) {
+        const int j = jn;
+        if (i - 1 >= l2) jn = mine[(i - 1) * kExpandBlock];
+        set_job_bit<NW>(sm, j);
+        int pk[MM];
+        job_row<MM>(sp, j, pk);
+        tail_step<MM>(q, pk);
 #pragma unroll
-      for (int k = 0; k < MM; ++k) wk[k] += pk[k];
-    }
-#pragma unroll
-    for (int k = 0; k < MM; ++k) {
-      S.p_hr[k * PH + p] = r[k];
-      S.p_hq[k * PH + p] = q[k];
-    }
-#pragma unroll
-    for (int w = 0; w < NW; ++w) S.p_hm[w * PH + p] = sm[w];
-    // selector features of the parent (the feature kernel's definition and
-    // fragment layout; used with m == MM so every index is a constant): per
-    // machine head (or R0), tail (or Q0), remaining work, head + work,
-    // work + tail, scaled
-    if constexpr (FEAT) {
-      int h[MM], tl[MM], rw[MM];
+        for (int k = 0; k < MM; ++k) wk[k] += pk[k];
+      }
 #pragma unroll
       for (int k = 0; k < MM; ++k) {
-        h[k] = l1 >= 0 ? r[k] : ld_const(T.R0, k);
-        tl[k] = l2 < n ? q[k] : ld_const(T.Q0, k);
-        rw[k] = ld_const(T.ptot, k) - wk[k];
+        S.p_hr[k * PH + p] = r[k];
+        S.p_hq[k * PH + p] = q[k];
       }
-      const int b = p >> 5, c = p & 31;
 #pragma unroll
-      for (int t = 0; t < (5 * MM + 7) / 8; ++t)
+      for (int w = 0; w < NW; ++w) S.p_hm[w * PH + p] = sm[w];
+      // selector features of the parent (the feature kernel's definition and
+      // fragment layout; used with m == MM so every index is a constant): per
+      // machine head (or R0), tail (or Q0), remaining work, head + work,
+      // work + tail, scaled
+      if constexpr (FEAT) {
+        int h[MM], tl[MM], rw[MM];
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          uint32_t w2[2] = {0u, 0u};
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int f = 8 * t + 4 * half + i, g = f / MM, k = f % MM;
-            if (f < 5 * MM) {
-              const int v = g == 0 ? h[k] : g == 1 ? tl[k] : g == 2 ? rw[k] : g == 3 ? h[k] + rw[k] : rw[k] + tl[k];
-              const float x = static_cast<float>(v) * S.x_inv_scale;
-              if (S.x_f32) S.x_f32[static_cast<size_t>(f) * S.x_ld + p] = x;
-              uint32_t u = __float_as_uint(x);
-              u += 0x7FFFu + ((u >> 16) & 1u);  // BF16, round to nearest even
-              w2[i >> 1] |= (u >> 16) << (16 * (i & 1));
-            }
-          }
-          if (S.x_frag)
-            S.x_frag[(static_cast<size_t>(b) * S.x_fsteps + t) * 64 + half * 32 + c] =
-                make_uint2(w2[0], w2[1]);
+        for (int k = 0; k < MM; ++k) {
+          h[k] = l1 >= 0 ? r[k] : ld_const(T.R0, k);
+          tl[k] = l2 < n ? q[k] : ld_const(T.Q0, k);
+          rw[k] = ld_const(T.ptot, k) - wk[k];
         }
+        const int b = p >> 5, c = p & 31;
+#pragma unroll
+        for (int t = 0; t < (5 * MM + 7) / 8; ++t)
+#pragma unroll
+          for (int half = 0; half < 2; ++half) {
+            uint32_t w2[2] = {0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int f = 8 * t + 4 * half + i, g = f / MM, k = f % MM;
+              if (f < 5 * MM) {
+                const int v = g == 0 ? h[k] : g == 1 ? tl[k] : g == 2 ? rw[k] : g == 3 ? h[k] + rw[k] : rw[k] + tl[k];
+                const float x = static_cast<float>(v) * S.x_inv_scale;
+                if (S.x_f32) S.x_f32[static_cast<size_t>(f) * S.x_ld + p] = x;
+                uint32_t u = __float_as_uint(x);
+                u += 0x7FFFu + ((u >> 16) & 1u);  // BF16, round to nearest even
+                w2[i >> 1] |= (u >> 16) << (16 * (i & 1));
+              }
+            }
+            if (S.x_frag)
+              S.x_frag[(static_cast<size_t>(b) * S.x_fsteps + t) * 64 + half * 32 + c] =
+                  make_uint2(w2[0], w2[1]);
+          }
+      }
     }
   }
   for (int d = 32; d >= 1; d >>= 1) work += __shfl_xor(work, d, 64);
