@@ -183,7 +183,7 @@ int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
  * launches them directly (-1 keeps the setting; default 0: both measure
  * the same, the rounds being bound by the device); rounds_per_sync
  * rounds run between host checks of the pool state (1..64, -1 keeps it;
- * default 8).  Bound-kernel timing (wsbb_search_profile) launches directly.
+ * default 4).  Bound-kernel timing (wsbb_search_profile) launches directly.
  * No counterpart in the reference (its host loop synchronises every pool,
  * explorer.cu:216-250). */
 int wsbb_search_options(wsbb_ctx *ctx, int use_graph, int rounds_per_sync);

@@ -65,7 +65,7 @@ struct wsbb_ctx {
   int pool_capacity = 1 << 24;  // device node-pool capacity (nodes)
   int profile_every = 0;         // time every k-th round's bound launch in wsbb_solve
   bool search_graph = false;     // replay search rounds from a captured hipGraph
-  int rounds_per_sync = 8;       // rounds queued between host status checks
+  int rounds_per_sync = 4;       // rounds queued between host status checks
   // autotuned shapes by batch size class (log2 of tiles), -1 = not tuned
   int tuned_groups[32];
   wsbb::SearchState* search = nullptr;
