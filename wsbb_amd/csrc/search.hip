@@ -1411,13 +1411,20 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   // the host waits on the current one, so the GPU never idles on a host
   // round trip; once a batch reports the search finished, the batch behind
   // it only repeats finished rounds (each returns at once) and is drained.
+  // timed rounds: every `every`-th round of the solve, counted across
+  // batches (the first round of a slot's batch is round base[slot])
+  long long queued = 0, base[2] = {0, 0};
   auto enqueue = [&](int slot) -> hipError_t {
     hipError_t le;
+    base[slot] = queued;
+    queued += rounds_per_sync;
     if (use_graph) {
       if ((le = hipGraphLaunch(S->graph, ctx->stream)) != hipSuccess) return le;
     } else {
       for (int r = 0; r < rounds_per_sync; ++r) {
-        hipEvent_t* ev = timing && r % every == 0 ? &S->ev[2 * (slot * rounds_per_sync + r)] : nullptr;
+        hipEvent_t* ev = timing && (base[slot] + r) % every == 0
+                             ? &S->ev[2 * (slot * rounds_per_sync + r)]
+                             : nullptr;
         if ((le = launch_round(ctx, A, kind, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr)) !=
             hipSuccess)
           return le;
@@ -1436,7 +1443,8 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     const long long live = S->host_ctrl[slot].stats[kStRounds] - prev_rounds;
     prev_rounds = S->host_ctrl[slot].stats[kStRounds];
     if (!timing) return le;
-    for (int r = 0; r < rounds_per_sync && r < live; r += every) {
+    for (int r = 0; r < rounds_per_sync && r < live; ++r) {
+      if ((base[slot] + r) % every != 0) continue;
       float t = 0;
       const int i = 2 * (slot * rounds_per_sync + r);
       if (hipEventElapsedTime(&t, S->ev[i], S->ev[i + 1]) == hipSuccess) {
