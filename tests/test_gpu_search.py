@@ -103,6 +103,27 @@ def test_graph_replay_matches_direct_launches(rounds):
                 assert res["stats"][k] == direct["stats"][k], (tid, k)
 
 
+@pytest.mark.parametrize("rounds,every", [(1, 8), (4, 8), (8, 8), (3, 5), (16, 4)])
+def test_bound_timing_every_kth_round(rounds, every):
+    """search_profile(k) times the bound launch of every k-th round of the
+    solve, counted across the host's batches of rounds: ceil(R / k) launches
+    for R rounds that did work, whatever the batch size; the timed solve
+    proves the same optimum with the same counters."""
+    p = oracle.taillard(11)[3]
+    with wsbb_amd.Engine(p) as eng:
+        eng.search_options(rounds_per_sync=rounds)
+        plain = eng.solve(pool_size=4096, max_batches=10)
+        eng.search_profile(every)
+        res = eng.solve(pool_size=4096, max_batches=10)
+        ms, timed = eng.search_profile(0)
+    R = res["stats"]["rounds"]
+    assert res["best"] == plain["best"] == 1582 and res["proven"]
+    for k in STATS:
+        assert res["stats"][k] == plain["stats"][k], k
+    assert timed == -(-R // every)
+    assert ms > 0
+
+
 def test_checkpoint_resume_continues_exactly():
     """A search stopped by its node budget, saved (pool, incumbent, counters)
     and resumed in a fresh context, in segments, visits exactly the nodes of
