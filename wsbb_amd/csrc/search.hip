@@ -667,7 +667,10 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
       const int first = simple ? 0 : side * u;
       for (int i = lane; i < nch; i += 64) {
         const int c = first + i;
-        const int j = S.c_desc[off + c] & 255;
+        // the i-th child of either side (and of a simple parent) places the
+        // job at position l1 + 1 + i, as expand numbered the descriptors:
+        // no descriptor load in front of the job's
+        const int j = l1 + 1 + i;
         s_cost[wave][i] = S.c_bound[off + c];
         s_job[wave][i] = S.p_perm[static_cast<size_t>(j) * S.PCAP + p];
       }
