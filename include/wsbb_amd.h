@@ -176,7 +176,31 @@ int wsbb_solve(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
  * explorer.cu:219-249). */
 int wsbb_search_profile(wsbb_ctx *ctx, int enable, double *bound_ms,
                         int64_t *bound_launches);
-/* Device node-pool capacity in nodes (default 2^24) for later solves. */
+/* Search trace, for tests and debugging.  With fn set, later wsbb_solve /
+ * wsbb_solve_resume calls run one round at a time (no graph, no rounds
+ * queued ahead) and, after every round that took parents, call fn on the
+ * calling thread with host copies of that round:
+ *   parents in pop order: permT [n][np] (position-major), l1, l2, cost [np];
+ *   children in generation order: desc [nc], bound [nc], where
+ *     desc = parent << 9 | side << 8 | j is the child of parent `parent`
+ *     that swaps position j with position l1 + 1 (side 0, BEGIN) or l2 - 1
+ *     (side 1, END), then applies limites_set (l2 - l1 == 2 -> l1 + 1);
+ *     both sides are listed, BEGIN children first (decompose_beginend.cpp
+ *     26-37, 114-138);
+ *   inc: the incumbent after the round.
+ * The buffers are valid during the call only.  fn = NULL turns it off.
+ * The reference prints per-pool statistics instead (explorer.cu:219-249). */
+typedef void (*wsbb_trace_fn)(void *user, int64_t round, int np,
+                              const uint8_t *permT, const int32_t *l1,
+                              const int32_t *l2, const int32_t *cost, int nc,
+                              const int32_t *desc, const int32_t *bound,
+                              int32_t inc);
+int wsbb_search_trace(wsbb_ctx *ctx, wsbb_trace_fn fn, void *user);
+/* Device node-pool capacity in nodes (default 2^24) for later solves.  A
+ * round that could leave more nodes than that on the pool is not started:
+ * the solve returns WSBB_EOVERFLOW with the pool, incumbent and counters as
+ * they were before that round, so wsbb_search_state followed by
+ * wsbb_solve_resume with a larger capacity continues exactly. */
 int wsbb_set_pool_capacity(wsbb_ctx *ctx, int nodes);
 /* Launch options of later solves: use_graph = 1 captures the kernels of
  * rounds_per_sync rounds in a hipGraph once per solve and replays it, 0

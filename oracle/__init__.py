@@ -16,6 +16,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 LB2, LB1 = 0, 1
+# search trace record types (oracle.h)
+TR_TAKEN, TR_PRUNED, TR_LEAF, TR_REPUSH = 0, 1, 2, 3
 NSTATS = 8
 STAT_NAMES = ("decomposed", "bounded", "leaves", "pruned_pop", "pruned_insert",
               "improvements", "max_pool", "rounds")
@@ -60,6 +62,11 @@ def lib():
                                     ctypes.c_int64, _i64p, _i32p]
         L.orc_solve_batched.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int64, _i64p, _i32p]
+        L.orc_solve_batched_ex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                           ctypes.c_int, _i64p, _i32p]
+        L.orc_trace_begin.argtypes = [_i32p, ctypes.c_int64, ctypes.c_int]
+        L.orc_trace_end.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -175,15 +182,42 @@ class Oracle:
         return dict(perm=cperm[:cnt], l1=cl1, l2=cl2, depth=cdepth, cost=ccost,
                     job=cjob, rank=crank, side=int(side[0]), nbounded=int(nb[0]))
 
-    def solve_dfs(self, kind=LB2, order=1, ub0=2**31 - 1, budget=0):
-        st = np.zeros(NSTATS, np.int64)
-        bp = np.zeros(self.n, np.int32)
-        best = lib().orc_solve_dfs(self.h, kind, order, ub0, budget, _p(st, _i64p), _p(bp))
-        return best, dict(zip(STAT_NAMES, st.tolist())), bp
+    def _traced(self, run, trace):
+        """Runs run() with the search trace on when trace is a record
+        capacity; returns (run's result, trace dict or None)."""
+        if not trace:
+            return run(), None
+        n = self.n
+        buf = np.zeros((int(trace), 2 * n + 8), np.int32)
+        lib().orc_trace_begin(_p(buf), int(trace), n)
+        try:
+            res = run()
+        finally:
+            cnt = lib().orc_trace_end()
+        if cnt > trace:
+            raise ValueError(f"trace needs {cnt} records, capacity {trace}")
+        r = buf[:cnt]
+        return res, dict(type=r[:, 0], round=r[:, 1], l1=r[:, 2], l2=r[:, 3], depth=r[:, 4],
+                         cost=r[:, 5], perm=r[:, 7:7 + n], ranks=r[:, 7 + n:8 + 2 * n])
 
-    def solve_batched(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
+    def solve_dfs(self, kind=LB2, order=1, ub0=2**31 - 1, budget=0, trace=0):
+        """CPU DFS explorer.  trace = record capacity: also return the
+        trace of pops (oracle.h) as a fourth value."""
         st = np.zeros(NSTATS, np.int64)
         bp = np.zeros(self.n, np.int32)
-        best = lib().orc_solve_batched(self.h, kind, pool_size, max_batches, ub0, budget,
-                                       _p(st, _i64p), _p(bp))
-        return best, dict(zip(STAT_NAMES, st.tolist())), bp
+        best, tr = self._traced(lambda: lib().orc_solve_dfs(
+            self.h, kind, order, ub0, budget, _p(st, _i64p), _p(bp)), trace)
+        res = (best, dict(zip(STAT_NAMES, st.tolist())), bp)
+        return res + (tr,) if trace else res
+
+    def solve_batched(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0,
+                      order=1, trace=0):
+        """Batched GPU-path explorer; order 1 = stack, 0 = ranks heap.
+        trace = record capacity: also return the trace as a fourth value."""
+        st = np.zeros(NSTATS, np.int64)
+        bp = np.zeros(self.n, np.int32)
+        best, tr = self._traced(lambda: lib().orc_solve_batched_ex(
+            self.h, kind, pool_size, max_batches, ub0, budget, order, _p(st, _i64p), _p(bp)),
+            trace)
+        res = (best, dict(zip(STAT_NAMES, st.tolist())), bp)
+        return res + (tr,) if trace else res

@@ -783,6 +783,51 @@ static void push_root(pool_t *pl, int n) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Search trace (test infrastructure): one record of 2n + 8 int32 per event, */
+/* [type, round, l1, l2, depth, cost, 0, perm[n], ranks[n+1]], in the order  */
+/* the explorer meets them.  Types: ORC_TR_TAKEN (popped and decomposed),    */
+/* ORC_TR_PRUNED (popped, bound not below the incumbent), ORC_TR_LEAF (leaf  */
+/* popped by the DFS explorer, or leaf child met by the batched explorer's   */
+/* prune loop, which never stores leaves), ORC_TR_REPUSH (popped by the      */
+/* batched explorer's collection loop, did not fit the sub-batch, pushed     */
+/* back).  Not re-entrant: one traced search at a time.                      */
+/* ------------------------------------------------------------------------ */
+static int32_t *g_trace;
+static int64_t g_trace_cap, g_trace_len;
+static int g_trace_n;
+
+void orc_trace_begin(int32_t *buf, int64_t cap_records, int n) {
+  g_trace = buf;
+  g_trace_cap = cap_records;
+  g_trace_len = 0;
+  g_trace_n = n;
+}
+int64_t orc_trace_end(void) {
+  int64_t r = g_trace_len;
+  g_trace = NULL;
+  g_trace_cap = g_trace_len = 0;
+  return r;
+}
+static void trace_rec(int type, int64_t round, int l1, int l2, int depth,
+                      int cost, const int *perm, const unsigned char *ranks) {
+  if (!g_trace) return;
+  if (g_trace_len < g_trace_cap) {
+    int n = g_trace_n;
+    int32_t *r = g_trace + (size_t)g_trace_len * (size_t)(2 * n + 8);
+    r[0] = type;
+    r[1] = (int32_t)round;
+    r[2] = l1;
+    r[3] = l2;
+    r[4] = depth;
+    r[5] = cost;
+    r[6] = 0;
+    for (int i = 0; i < n; i++) r[7 + i] = perm[i];
+    for (int i = 0; i <= n; i++) r[7 + n + i] = ranks ? ranks[i] : 0;
+  }
+  g_trace_len++; /* counts past the capacity, so the caller can tell */
+}
+
+/* ------------------------------------------------------------------------ */
 /* CPU explorer: explorer::_explore_cpu (explorer.cu:92-148)                 */
 /* ------------------------------------------------------------------------ */
 
@@ -800,6 +845,8 @@ int orc_solve_dfs(const orc_inst *h, int kind, int order, int ub0,
   orc_expansion *x = (orc_expansion *)malloc(sizeof(orc_expansion));
   while (pool_pop(&pl, &cur)) {
     if (cur.depth == n) { /* bb_problem_calculator::leaf 54-56 */
+      trace_rec(ORC_TR_LEAF, st[0], cur.l1, cur.l2, cur.depth, cur.cost,
+                cur.perm, cur.ranks);
       st[2]++;
       /* solutions::insert (solutions.cpp:45-65): strictly better only */
       if (!have ? cur.cost < best : cur.cost < best) {
@@ -813,6 +860,8 @@ int orc_solve_dfs(const orc_inst *h, int kind, int order, int ub0,
         exhausted = 1;
         break;
       }
+      trace_rec(ORC_TR_TAKEN, st[0], cur.l1, cur.l2, cur.depth, cur.cost,
+                cur.perm, cur.ranks);
       expand(h, kind, cur.perm, cur.l1, cur.l2, cur.depth, x);
       st[0]++;
       st[1] += x->nbounded;
@@ -832,6 +881,8 @@ int orc_solve_dfs(const orc_inst *h, int kind, int order, int ub0,
       }
       if (pool_count(&pl) > st[6]) st[6] = pool_count(&pl);
     } else {
+      trace_rec(ORC_TR_PRUNED, st[0], cur.l1, cur.l2, cur.depth, cur.cost,
+                cur.perm, cur.ranks);
       st[3]++;
     }
   }
@@ -855,11 +906,18 @@ int orc_solve_dfs(const orc_inst *h, int kind, int order, int ub0,
 int orc_solve_batched(const orc_inst *h, int kind, int pool_size,
                       int max_batches, int ub0, int64_t budget,
                       int64_t *stats, int *best_perm) {
+  return orc_solve_batched_ex(h, kind, pool_size, max_batches, ub0, budget, 1,
+                              stats, best_perm);
+}
+
+int orc_solve_batched_ex(const orc_inst *h, int kind, int pool_size,
+                         int max_batches, int ub0, int64_t budget, int order,
+                         int64_t *stats, int *best_perm) {
   int n = h->n;
   if (pool_size - 1 <= 2 * n || max_batches < 1) return -2;
   int64_t st[ORC_NSTATS] = {0};
   pool_t pl;
-  pool_init(&pl, n, 1);
+  pool_init(&pl, n, order);
   push_root(&pl, n);
   int best = ub0;
   int exhausted = 0;
@@ -879,6 +937,8 @@ int orc_solve_batched(const orc_inst *h, int kind, int pool_size,
         if (cur.cost < best) { /* tree::bound, tree.cpp:247-250 */
           int fils = 2 * (n - cur.depth);
           if (size1 + fils < pool_size - 1) { /* problems::push_back1 */
+            trace_rec(ORC_TR_TAKEN, st[7], cur.l1, cur.l2, cur.depth,
+                      cur.cost, cur.perm, cur.ranks);
             if (np == pcap) {
               pcap *= 2;
               par = (popped *)realloc(par, sizeof(popped) * pcap);
@@ -887,10 +947,14 @@ int orc_solve_batched(const orc_inst *h, int kind, int pool_size,
             size1 += fils;
             cnt++;
           } else {
+            trace_rec(ORC_TR_REPUSH, st[7], cur.l1, cur.l2, cur.depth,
+                      cur.cost, cur.perm, cur.ranks);
             pool_repush(&pl, &cur);
             break;
           }
         } else {
+          trace_rec(ORC_TR_PRUNED, st[7], cur.l1, cur.l2, cur.depth,
+                    cur.cost, cur.perm, cur.ranks);
           st[3]++;
         }
       }
@@ -920,6 +984,8 @@ int orc_solve_batched(const orc_inst *h, int kind, int pool_size,
       int kept = 0;
       for (int c = 0; c < cnt; c++) { /* worst first */
         if (v[c]->depth == n) {
+          trace_rec(ORC_TR_LEAF, st[7], v[c]->l1, v[c]->l2, v[c]->depth,
+                    v[c]->cost, v[c]->perm, par[i].ranks);
           st[2]++;
           if (v[c]->cost < best) {
             best = v[c]->cost;

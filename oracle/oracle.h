@@ -101,6 +101,22 @@ int orc_solve_dfs(const orc_inst *h, int kind, int order, int ub0,
 int orc_solve_batched(const orc_inst *h, int kind, int pool_size,
                       int max_batches, int ub0, int64_t budget,
                       int64_t *stats, int *best_perm);
+/* the same with the pool order chosen: 0 = lexicographic-ranks heap (the
+ * reference's priority_queue), 1 = LIFO stack (the default above) */
+int orc_solve_batched_ex(const orc_inst *h, int kind, int pool_size,
+                         int max_batches, int ub0, int64_t budget, int order,
+                         int64_t *stats, int *best_perm);
+
+/* --- search trace (test infrastructure) ---
+ * Between orc_trace_begin and orc_trace_end the explorers append one record
+ * of 2n + 8 int32 per event to buf (at most cap_records are stored; the
+ * count returned by orc_trace_end includes those past the capacity):
+ * [type, round, l1, l2, depth, cost, 0, perm[n], ranks[n+1]]; round is the
+ * decomposition count (DFS) or the round number (batched).  A leaf met by
+ * the batched prune loop carries its parent's ranks. */
+enum { ORC_TR_TAKEN = 0, ORC_TR_PRUNED = 1, ORC_TR_LEAF = 2, ORC_TR_REPUSH = 3 };
+void orc_trace_begin(int32_t *buf, int64_t cap_records, int n);
+int64_t orc_trace_end(void);
 
 #ifdef __cplusplus
 }

@@ -325,3 +325,108 @@ def test_small_instances_against_brute_force_optimum():
         assert o.solve_dfs()[0] == opt
         assert o.solve_dfs(kind=oracle.LB1)[0] == opt
         assert o.solve_batched(pool_size=2 * n + 2, max_batches=2)[0] == opt
+
+
+# ---- search invariants on the explorers' traces --------------------------
+# These pin the search rules by properties a misreading would break, not by
+# restating them: the pool order (tree.cpp:37-46) through the rank strings
+# popped, the counters through conservation of pushed and popped nodes, and
+# the strictly-better incumbent (solutions.cpp:45-65) through tied leaves.
+from tracecheck import lex_nondecreasing, taken  # noqa: E402
+
+
+def _free(l1, l2):
+    return l2 - l1 - 1
+
+
+@pytest.mark.parametrize("tid", [1, 4, 7])
+def test_dfs_pops_follow_rank_order(tid):
+    """The CPU explorer's priority_queue pops the smallest rank string; a
+    child's string is its parent's with ranks[depth] raised from 0, so the
+    faithful heap pops a non-decreasing sequence, and the LIFO stack the
+    device's pool follows pops exactly the same sequence."""
+    o = oracle.Oracle(oracle.taillard(tid)[3])
+    bh, sth, _, th = o.solve_dfs(order=0, trace=200_000)
+    bs, sts, _, ts = o.solve_dfs(order=1, trace=200_000)
+    assert lex_nondecreasing(th["ranks"])
+    assert bh == bs and sth == sts
+    for k in th:
+        assert np.array_equal(th[k], ts[k]), k
+
+
+@pytest.mark.parametrize("tid", [1, 4, 7])
+def test_dfs_counters_conserve_nodes(tid):
+    """Every node pushed is popped once: 1 + the chosen side of every
+    decomposed parent (u children, or 2 leaves for a parent at depth n-2)
+    equals decomposed + pruned at pop + leaves; children bounded are both
+    sides, 2u (or 2) per parent."""
+    p = oracle.taillard(tid)[3]
+    n = p.shape[1]
+    o = oracle.Oracle(p)
+    _, st, _, tr = o.solve_dfs(order=0, trace=200_000)
+    tk = taken(tr)
+    u = _free(tk["l1"], tk["l2"])
+    simple = tk["depth"] == n - 2
+    pushed = 1 + int(np.where(simple, 2, u).sum())
+    popped = st["decomposed"] + st["pruned_pop"] + st["leaves"]
+    assert pushed == popped == tr["type"].shape[0]
+    assert st["bounded"] == int(np.where(simple, 2, 2 * u).sum())
+    assert st["decomposed"] == tk["type"].shape[0]
+    assert st["leaves"] == int((tr["type"] == oracle.TR_LEAF).sum())
+
+
+@pytest.mark.parametrize("tid,pool,nb", [(1, 4096, 1), (3, 300, 3), (2, 4096, 10)])
+def test_batched_rounds_take_the_smallest_rank_strings(tid, pool, nb):
+    """The batched explorer (explorer.cu:166-208) takes each round's parents
+    from the priority_queue one by one: within a round the rank strings
+    popped increase strictly, and the stack form pops the same events as the
+    faithful heap.  Conservation: every node pushed (the root and the kept
+    children of inner parents) is taken or pruned at pop, leaves are the two
+    children of each parent at depth n-2, children bounded are 2u or 2."""
+    p = oracle.taillard(tid)[3]
+    n = p.shape[1]
+    o = oracle.Oracle(p)
+    bh, sth, _, th = o.solve_batched(pool_size=pool, max_batches=nb, order=0, trace=3_000_000)
+    bs, sts, _, ts = o.solve_batched(pool_size=pool, max_batches=nb, order=1, trace=3_000_000)
+    assert bh == bs and sth == sts
+    for k in th:
+        assert np.array_equal(th[k], ts[k]), k
+    tk = taken(th)
+    for r in np.unique(tk["round"]):
+        assert lex_nondecreasing(tk["ranks"][tk["round"] == r], strict=True)
+    u = _free(tk["l1"], tk["l2"])
+    simple = tk["depth"] == n - 2
+    kept = int(np.where(simple, 0, u).sum()) - sth["pruned_insert"]
+    assert sth["decomposed"] + sth["pruned_pop"] == 1 + kept
+    assert sth["leaves"] == 2 * int(simple.sum()) == int((th["type"] == oracle.TR_LEAF).sum())
+    assert sth["bounded"] == int(np.where(simple, 2, 2 * u).sum())
+    # a last round may pop only pruned candidates: it counts, takes nothing
+    assert sth["rounds"] - int(tk["round"].max()) in (0, 1)
+
+
+def _tie_instances():
+    yield np.full((3, 7), 4, np.int32)          # every leaf has the same makespan
+    rng = np.random.default_rng(11)
+    for _ in range(6):
+        n, m = int(rng.integers(6, 9)), int(rng.integers(2, 4))
+        yield rng.integers(1, 4, size=(m, n)).astype(np.int32)
+
+
+def test_tied_leaves_keep_the_first_found():
+    """solutions::insert stores a leaf only if it is strictly better than
+    every stored one: among leaves of the optimal makespan the incumbent is
+    the first met, in both explorers.  The instances have several optimal
+    leaves, so replacing on ties (the last one met) would fail."""
+    differs = 0
+    for p in _tie_instances():
+        o = oracle.Oracle(p)
+        for run in (lambda: o.solve_dfs(order=0, trace=500_000),
+                    lambda: o.solve_batched(pool_size=64, max_batches=2, trace=500_000)):
+            best, _, bp, tr = run()
+            leaves = tr["type"] == oracle.TR_LEAF
+            opt = np.flatnonzero(leaves & (tr["cost"] == best))
+            assert opt.size > 0
+            assert np.array_equal(bp, tr["perm"][opt[0]])
+            assert o.cmax(bp) == best
+            differs += not np.array_equal(tr["perm"][opt[0]], tr["perm"][opt[-1]])
+    assert differs > 0

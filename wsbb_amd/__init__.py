@@ -28,6 +28,11 @@ _f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
+# wsbb_trace_fn: (user, round, np, permT, l1, l2, cost, nc, desc, bound, inc)
+_TRACE_FN = ctypes.CFUNCTYPE(None, _vp, ctypes.c_int64, ctypes.c_int, _u8p, _i32p, _i32p, _i32p,
+                             ctypes.c_int, _i32p, _i32p, ctypes.c_int32)
+
+
 class WsbbError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -71,6 +76,7 @@ def _load():
                                 ctypes.c_int),
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_search_options": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_search_trace": ([_vp, _TRACE_FN, _vp], ctypes.c_int),
         "wsbb_set_int16": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_multi_create": ([ctypes.c_int, ctypes.POINTER(_vp), _i32p, _i32p, ctypes.c_int,
                                ctypes.POINTER(_vp)], ctypes.c_int),
@@ -398,6 +404,34 @@ class Engine:
                                                _p(probs, _f32p), _p(tsel, _u32p)),
                "infer_parents")
         return probs, tsel
+
+    def search_trace(self, on=True):
+        """Record every round of later solves (wsbb_search_trace): returns
+        the list the rounds are appended to, one dict per round with the
+        parents in pop order (permT [n, np], l1, l2, cost), the children in
+        generation order (desc, bound) and the incumbent after the round.
+        on=False turns it off."""
+        if not on:
+            _check(lib.wsbb_search_trace(self.h, _TRACE_FN(), None), "search_trace")
+            self._trace_cb = None
+            return None
+        rounds = []
+        n = self.n
+
+        def cb(_user, rnd, np_, permT, l1, l2, cost, nc, desc, bound, inc):
+            def arr(ptr, cnt, dt):
+                return np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True) \
+                    if cnt else np.zeros(0, dt)
+            rounds.append(dict(
+                round=int(rnd),
+                permT=arr(permT, n * np_, np.uint8).reshape(n, np_),
+                l1=arr(l1, np_, np.int32), l2=arr(l2, np_, np.int32),
+                cost=arr(cost, np_, np.int32), desc=arr(desc, nc, np.int32),
+                bound=arr(bound, nc, np.int32), inc=int(inc)))
+
+        self._trace_cb = _TRACE_FN(cb)
+        _check(lib.wsbb_search_trace(self.h, self._trace_cb, None), "search_trace")
+        return rounds
 
     def solve(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
         best = ctypes.c_int32()

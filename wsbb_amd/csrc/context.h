@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/wsbb_amd.h"
 #include "device.h"
 #include "instance.h"
 
@@ -66,6 +67,8 @@ struct wsbb_ctx {
   int profile_every = 0;         // time every k-th round's bound launch in wsbb_solve
   bool search_graph = false;     // replay search rounds from a captured hipGraph
   int rounds_per_sync = 4;       // rounds queued between host status checks
+  wsbb_trace_fn trace_fn = nullptr;  // per-round search trace (one round per sync)
+  void* trace_user = nullptr;
   // autotuned shapes by batch size class (log2 of tiles), -1 = not tuned
   int tuned_groups[32];
   wsbb::SearchState* search = nullptr;

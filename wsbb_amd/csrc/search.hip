@@ -393,7 +393,14 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
   if (tid == 0) {
     C->n_fb = 0;
-    if (np > S.PCAP || nc > S.CCAP) {
+    // Overflow is decided here, before anything is committed: the round
+    // keeps at most nc children on top of the top - consumed nodes left
+    // below its parents, so if those fit, settle and push cannot run out of
+    // stack.  An overflowing round leaves the stack, incumbent and counters
+    // exactly as they were before it, and a search resumed from that state
+    // (with more capacity) visits the nodes an uninterrupted one would.
+    if (np > S.PCAP || nc > S.CCAP ||
+        static_cast<long long>(top - consumed) + nc > S.CAP) {
       C->status = kOverflow;
       C->n_parents = 0;
       C->n_children = 0;
@@ -869,13 +876,10 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
     if (bl >= 0) C->inc = run_min;
     C->push_base = sbase;
     C->push_total = run_kept;
-    if (static_cast<long long>(sbase) + run_kept > S.CAP) {
-      C->status = kOverflow;
-    } else {
-      C->stack_size = sbase + run_kept;
-      atomicMax(reinterpret_cast<unsigned long long*>(&C->stats[kStMaxPool]),
-                static_cast<unsigned long long>(sbase + run_kept));
-    }
+    // fits: select admitted the round only with room for all its children
+    C->stack_size = sbase + run_kept;
+    atomicMax(reinterpret_cast<unsigned long long*>(&C->stats[kStMaxPool]),
+              static_cast<unsigned long long>(sbase + run_kept));
   }
   // the leaf that set the incumbent: parent permutation with its swap
   if (bl >= 0 && tid < n) {
@@ -1296,6 +1300,72 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   return hipGetLastError();
 }
 
+// final state of a solve from its last control block
+static int finish_solve(const wsbb_ctx* ctx, const Ctrl& H, int32_t* best, int32_t* best_perm,
+                        int64_t* stats) {
+  const int n = ctx->T.n;
+  if (best) *best = H.inc;
+  if (best_perm) {
+    for (int i = 0; i < n; ++i) best_perm[i] = H.have_perm ? H.best_perm[i] : -1;
+  }
+  if (stats)
+    for (int i = 0; i < WSBB_NSTATS; ++i) stats[i] = H.stats[i];
+  if (H.status == kOverflow) {
+    set_error("device node pool overflow; raise the pool capacity");
+    return WSBB_EOVERFLOW;
+  }
+  if (H.status == kBudget) {
+    set_error("node budget exhausted");
+    return WSBB_EBUDGET;
+  }
+  return WSBB_OK;
+}
+
+// One round at a time, each followed by host copies of its parents,
+// children and bounds handed to the trace callback.
+static int solve_traced(wsbb_ctx* ctx, const SArgs& A, int kind, int32_t* best,
+                        int32_t* best_perm, int64_t* stats) {
+  SearchState* S = ctx->search;
+  const int n = ctx->T.n;
+  std::vector<uint8_t> permT;
+  std::vector<int32_t> l1, l2, cost, desc, bound;
+  hipError_t e;
+  for (long long round = 0;; ++round) {
+    if ((e = launch_round(ctx, A, kind, nullptr, nullptr)) != hipSuccess)
+      return hip_fail(e, "round launch");
+    Ctrl& H = S->host_ctrl[0];
+    e = hipMemcpyAsync(&H, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "trace control copy");
+    const int np = H.n_parents, nc = H.n_children;
+    if (np > 0) {
+      permT.resize(static_cast<size_t>(n) * np);
+      l1.resize(np);
+      l2.resize(np);
+      cost.resize(np);
+      desc.resize(nc);
+      bound.resize(nc);
+      e = hipMemcpy2DAsync(permT.data(), np, A.p_perm, A.PCAP, np, n, hipMemcpyDeviceToHost,
+                           ctx->stream);
+      const std::pair<int32_t*, const int*> rows[] = {{l1.data(), A.p_l1}, {l2.data(), A.p_l2},
+                                                      {cost.data(), A.p_cost}};
+      for (const auto& r : rows)
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(r.first, r.second, 4 * size_t(np), hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess && nc > 0)
+        e = hipMemcpyAsync(desc.data(), A.c_desc, 4 * size_t(nc), hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess && nc > 0)
+        e = hipMemcpyAsync(bound.data(), A.c_bound, 4 * size_t(nc), hipMemcpyDeviceToHost,
+                           ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return hip_fail(e, "trace copy");
+      ctx->trace_fn(ctx->trace_user, round, np, permT.data(), l1.data(), l2.data(), cost.data(),
+                    nc, desc.data(), bound.data(), H.inc);
+    }
+    if (H.status != kRunning) return finish_solve(ctx, H, best, best_perm, stats);
+  }
+}
+
 int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
           int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
           int cap_nodes, int rounds_per_sync, const ResumeState* resume) {
@@ -1337,13 +1407,33 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
       set_error("saved pool larger than the device pool capacity");
       return WSBB_EOVERFLOW;
     }
+    // Every pool node the device makes has at least two free jobs (a child
+    // left with one is completed by limites_set, and leaves are evaluated,
+    // never stored), and each column is a permutation of 0..n-1.
     for (int k = 0; k < cnt; ++k) {
       const int a = resume->l1[k], b = resume->l2[k];
-      bool ok = a >= -1 && b <= n && a < b;
-      for (int i = 0; ok && i < n; ++i) ok = resume->permT[static_cast<size_t>(i) * cnt + k] < n;
+      bool ok = a >= -1 && b <= n && b - a >= 3;
+      uint64_t seen[2] = {0, 0};
+      for (int i = 0; ok && i < n; ++i) {
+        const int j = resume->permT[static_cast<size_t>(i) * cnt + k];
+        ok = j < n && !((seen[j >> 6] >> (j & 63)) & 1u);
+        if (ok) seen[j >> 6] |= uint64_t(1) << (j & 63);
+      }
       if (!ok) {
-        set_error("saved pool holds a node with bad limits or job ids");
+        set_error("saved pool node " + std::to_string(k) +
+                  " has fewer than two free jobs or is not a permutation of 0..n-1");
         return WSBB_EINVAL;
+      }
+    }
+    if (resume->best_perm && resume->best_perm[0] >= 0) {
+      uint64_t seen[2] = {0, 0};
+      for (int i = 0; i < n; ++i) {
+        const int j = resume->best_perm[i];
+        if (j < 0 || j >= n || ((seen[j >> 6] >> (j & 63)) & 1u)) {
+          set_error("incumbent permutation is not a permutation of 0..n-1");
+          return WSBB_EINVAL;
+        }
+        seen[j >> 6] |= uint64_t(1) << (j & 63);
       }
     }
     Ctrl h{};
@@ -1370,6 +1460,7 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "upload of the saved pool");
   }
+  if (ctx->trace_fn) return solve_traced(ctx, A, kind, best, best_perm, stats);
   const int every = ctx->profile_every;
   const bool timing = every > 0;
   if (timing && static_cast<int>(S->ev.size()) < 4 * rounds_per_sync) {
@@ -1471,22 +1562,7 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     }
     cur ^= 1;
   }
-  const Ctrl& H = S->host_ctrl[cur];
-  if (best) *best = H.inc;
-  if (best_perm) {
-    for (int i = 0; i < n; ++i) best_perm[i] = H.have_perm ? H.best_perm[i] : -1;
-  }
-  if (stats)
-    for (int i = 0; i < WSBB_NSTATS; ++i) stats[i] = H.stats[i];
-  if (H.status == kOverflow) {
-    set_error("device node pool overflow; raise the pool capacity");
-    return WSBB_EOVERFLOW;
-  }
-  if (H.status == kBudget) {
-    set_error("node budget exhausted");
-    return WSBB_EBUDGET;
-  }
-  return WSBB_OK;
+  return finish_solve(ctx, S->host_ctrl[cur], best, best_perm, stats);
 }
 
 }  // namespace wsbb
@@ -1572,6 +1648,16 @@ extern "C" int wsbb_solve_resume(wsbb_ctx* ctx, int kind, int pool_size, int max
   const wsbb::ResumeState R{permT, l1, l2, cost, count, stats_in, best_perm_in};
   return wsbb::solve(ctx, kind, pool_size, max_batches, inc, budget, best, best_perm, stats,
                      ctx->pool_capacity, ctx->rounds_per_sync, &R);
+}
+
+extern "C" int wsbb_search_trace(wsbb_ctx* ctx, wsbb_trace_fn fn, void* user) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  ctx->trace_fn = fn;
+  ctx->trace_user = fn ? user : nullptr;
+  return WSBB_OK;
 }
 
 extern "C" int wsbb_search_options(wsbb_ctx* ctx, int use_graph, int rounds_per_sync) {
