@@ -79,7 +79,9 @@ int wsbb_precompute(const int32_t *p, int n, int m, int32_t *pair_a,
  * 911-927, 953-956) together with gpu::init / allocate_on_device /
  * copy_to_device (src/gpu.cu:126-162, 620-642): precomputes the instance
  * tables on the host and uploads them to `device`.  Limits: 2 <= n <= 128,
- * 2 <= m <= 24, processing times in [0, 65535]. */
+ * 2 <= m <= 24, processing times in [0, 65535], and the path bound
+ * sum_j max_k p[k][j] + sum_k max_j p[k][j] (which no bound value exceeds)
+ * below 2^19 = 524288. */
 typedef struct wsbb_ctx wsbb_ctx;
 int wsbb_create(const int32_t *p, int n, int m, int device, wsbb_ctx **out);
 int wsbb_destroy(wsbb_ctx *ctx);

@@ -41,45 +41,76 @@ __device__ __forceinline__ uint32_t bitsel(uint32_t mask, uint32_t a, uint32_t b
   return (mask & a) | (~mask & b);  // v_bfi_b32
 }
 
-// four Johnson steps (one scalar-load group)
+// Johnson steps in the int32 offset form.  With S the running sum of p_b
+// over the unscheduled jobs passed, the scan keeps D = T0 - S and
+// U = t1 - S, both scaled by 32; an unscheduled job's step t0 += p_a;
+// t1 = max(t1, t0 + lag) + p_b (heuristiqueCmax, bound_flowshop.cpp:64-77)
+// becomes
+//   D += 32 (p_a - p_b);  U = max(U, D + 32 (lag + p_b))
+// and a scheduled job (flag f = 0) leaves D and gives max(U, D) = U, since
+// T0 <= t1 throughout.  The second constant also carries the job's bit
+// offset in its five low bits (a bit-field extract reads only those), so a
+// step is one extract (plus a word select for n > 32), two 24-bit
+// multiply-adds and a max; the offset adds less than 32 to a candidate,
+// which U >> 5 drops.  At the end S = rem(b): t1 = (U >> 5) + rem(b) and
+// T0 = (D >> 5) + rem(b).
 template <int NW>
-struct StepGroup {
-  int4 rec[4];   // {job, p_a, p_b, kBias + lag + p_b}
-  uint2 sel[4];  // word-select masks of the job (NW > 1)
-  __device__ __forceinline__ void load(const DevTables& T, size_t step) {
+struct Rec32 {
+  // dwords per step: the two constants, then one word mask per selection
+  // level (every dword of a batch is used, so it loads as wide scalar loads)
+  static constexpr int RS = NW == 1 ? 2 : (NW == 2 ? 3 : 4);
+  static constexpr int GW = 4 * RS;  // dwords per four-step group
+  // groups per scalar-load batch: at most 32 SGPRs of records per wait
+  static constexpr int CH = 32 / GW;
+};
+
+template <int NW, int CH>
+struct Chunk32 {
+  int w[CH * Rec32<NW>::GW];
+  __device__ __forceinline__ void load(const int* p) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) rec[u] = ld_const4(T.steps, step + u);
-    if constexpr (NW > 1) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) sel[u] = ld_const2u(T.wsel, step + u);
-    }
+    for (int i = 0; i < CH * Rec32<NW>::GW; ++i) w[i] = ld_const(p, i);
   }
 };
 
-// four Johnson steps of one pair: unscheduled job: t0 += p_a;
-// t1 = max(t1, t0 + lag) + p_b.  A scheduled job (f = 0) leaves both clocks
-// unchanged, since the biased t0 stays below t1.
 template <int NW>
-__device__ __forceinline__ void step4(const StepGroup<NW>& g, const uint32_t (&um)[NW],
-                                      int& T0, int& t1) {
+__device__ __forceinline__ void step32(const int* r, const uint32_t (&um)[NW], int& D, int& U) {
+  uint32_t word;
+  if constexpr (NW == 1) {
+    word = um[0];
+  } else if constexpr (NW == 2) {
+    word = bitsel(static_cast<uint32_t>(r[2]), um[1], um[0]);
+  } else {
+    const uint32_t w01 = bitsel(static_cast<uint32_t>(r[2]), um[1], um[0]);
+    const uint32_t w23 = bitsel(static_cast<uint32_t>(r[2]), um[3], um[2]);
+    word = bitsel(static_cast<uint32_t>(r[3]), w23, w01);
+  }
+  const uint32_t f = __builtin_amdgcn_ubfe(word, static_cast<uint32_t>(r[1]), 1u);
+  D = __mul24(static_cast<int>(f), r[0]) + D;
+  U = max(U, static_cast<int>(__umul24(f, static_cast<uint32_t>(r[1])) + static_cast<uint32_t>(D)));
+}
+
+template <int NW>
+__device__ __forceinline__ void group32(const int* g, const uint32_t (&um)[NW], int& D, int& U) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int4 qq = g.rec[u];
-    uint32_t word;
-    if constexpr (NW == 1) {
-      word = um[0];
-    } else if constexpr (NW == 2) {
-      word = bitsel(g.sel[u].x, um[1], um[0]);
-    } else {
-      const uint32_t w01 = bitsel(g.sel[u].x, um[1], um[0]);
-      const uint32_t w23 = bitsel(g.sel[u].x, um[3], um[2]);
-      word = bitsel(g.sel[u].y, w23, w01);
-    }
-    const uint32_t f = __builtin_amdgcn_ubfe(word, static_cast<uint32_t>(qq.x), 1u);
-    T0 = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.y)) + T0);
-    const int x = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.z)) + t1);
-    const int y = static_cast<int>(__umul24(f, static_cast<uint32_t>(qq.w)) + T0);
-    t1 = max(x, y);
+  for (int k = 0; k < 4; ++k) step32<NW>(g + k * Rec32<NW>::RS, um, D, U);
+}
+
+// One pair's scan over its gpp groups, CH groups per scalar-load batch
+// (a batch may read past the pair's end: the table holds 16 zero steps
+// after the last pair).
+template <int NW>
+__device__ __forceinline__ void scan_pair32(const DevTables& T, int s, const uint32_t (&um)[NW],
+                                            int& D, int& U) {
+  constexpr int GW = Rec32<NW>::GW, CH = Rec32<NW>::CH;
+  const int gpp = T.n4 >> 2;
+  const int* gp = T.steps32 + static_cast<size_t>(s) * gpp * GW;
+  for (int k = 0; k < gpp; k += CH) {
+    Chunk32<NW, CH> c;
+    c.load(gp + static_cast<size_t>(k) * GW);
+#pragma unroll
+    for (int h = 0; h < CH; ++h)
+      if (k + h < gpp) group32<NW>(c.w + h * GW, um, D, U);
   }
 }
 
@@ -97,14 +128,19 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   // rows m..MM-1 of the staged times are zero: the recurrences then run
   // over all MM machines without per-machine guards, and the padded heads
   // and tails never exceed the real ones
-  const int mn4 = (MM * n + 3) & ~3;
-  int32_t* sp = lds;                                 // [n][MM]
+  const int mn4 = MM * (n + 3);
+  int32_t* sp = lds;  // [n][MM] times, then rows n, n+1, n+2: R0, Q0, row totals
   int4* part = reinterpret_cast<int4*>(lds + mn4);   // [blockDim] partials
   // [n][64 * tiles per block] bytes: the permutation columns of the block's
   // tiles, staged once per tile so the head/tail recurrences read LDS
   // instead of waiting on one global load per position
   uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);
   stage_times<MM>(T, sp);
+  for (int i = threadIdx.x; i < 3 * MM; i += blockDim.x) {
+    const int c = i / MM, k = i - c * MM;
+    const int32_t* row = c == 0 ? T.R0 : (c == 1 ? T.Q0 : T.ptot);  // zero past machine m
+    sp[(n + c) * MM + k] = row[k];
+  }
   __syncthreads();
 
   const int G = A.groups;
@@ -122,7 +158,6 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   const int nbt = (ntiles + tpb - 1) / tpb;
   int inc = 0;
   if constexpr (!LEAN) inc = (A.inc_dev != nullptr) ? *A.inc_dev : 0;
-  const int gpp = T.n4 >> 2;  // step groups per pair
 
   uint32_t valid_bits[NW];
 #pragma unroll
@@ -176,23 +211,64 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     // below yields the same longest-path values for all machines at once.
     // q[k] = tail of machine k; q[k] for k >= m stays 0, which is the
     // boundary of the reversed recurrence.
-    int r[MM], q[MM];
+    // r: heads of the prefix; ew: remaining work of the free jobs (row total
+    // minus the scheduled jobs) + tail of the suffix (Q0 without suffix),
+    // what a pair ending on machine k adds to t1 - S.  The tails are folded
+    // into ew as soon as the suffix is done, so at most three per-machine
+    // arrays are live at once.  qx keeps the tails for the features.
+    int r[MM], ew[MM];
+    int qx[EXTRA ? MM : 1];
     uint32_t sm[NW];
     if (from_parent) {
       // every lane reads a child's state (padding lanes the tile's last
       // child, their results unused), so the arrays are assigned on this
       // path without a zero fill held in registers beside them
-      if constexpr (Src::kState) src.template state<MM, NW>(min(node, N - 1), sp, r, q, sm);
+      if constexpr (Src::kState) {
+        int q[MM], sc[MM];
+        src.template state<MM, NW>(min(node, N - 1), sp, r, q, sm, sc);
+        int tot[MM], q0[MM];
+        job_row<MM>(sp, n + 2, tot);
+        job_row<MM>(sp, n + 1, q0);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+          ew[k] = tot[k] - sc[k] + (d2 ? q[k] : q0[k]);
+          if constexpr (EXTRA) qx[k] = q[k];
+        }
+      }
     } else {
       // a zero the compiler cannot hoist: hoisted, the fill below became a
       // block of registers held across the tile loop
       int zero;
       asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+      job_row<MM>(sp, n + 2, ew);  // row totals
 #pragma unroll
-      for (int k = 0; k < MM; ++k) r[k] = q[k] = zero;
+      for (int k = 0; k < MM; ++k) r[k] = zero;
 #pragma unroll
       for (int w = 0; w < NW; ++w) sm[w] = 0;
-      // the next job index is read while the current row is processed
+      {
+        int q[MM];
+#pragma unroll
+        for (int k = 0; k < MM; ++k) q[k] = zero;
+        // the next job index is read while the current row is processed
+        int jn = d2 > 0 ? perm_at(mine, bstride, sw0, sw1, n - 1) : 0;
+        for (int i = n - 1; i >= l2; --i) {
+          const int j = jn;
+          if (i - 1 >= l2) jn = perm_at(mine, bstride, sw0, sw1, i - 1);
+          set_job_bit<NW>(sm, j);
+          int pk[MM];
+          job_row<MM>(sp, j, pk);
+          tail_step<MM>(q, pk);
+#pragma unroll
+          for (int k = 0; k < MM; ++k) ew[k] -= pk[k];
+        }
+        int q0[MM];
+        job_row<MM>(sp, n + 1, q0);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+          ew[k] += d2 ? q[k] : q0[k];
+          if constexpr (EXTRA) qx[k] = q[k];
+        }
+      }
       int jn = d1 > 0 ? perm_at(mine, bstride, sw0, sw1, 0) : 0;
       for (int i = 0; i < d1; ++i) {
         const int j = jn;
@@ -201,52 +277,55 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
         int pk[MM];
         job_row<MM>(sp, j, pk);
         head_step<MM>(r, pk);
-      }
-      jn = d2 > 0 ? perm_at(mine, bstride, sw0, sw1, n - 1) : 0;
-      for (int i = n - 1; i >= l2; --i) {
-        const int j = jn;
-        if (i - 1 >= l2) jn = perm_at(mine, bstride, sw0, sw1, i - 1);
-        set_job_bit<NW>(sm, j);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-        tail_step<MM>(q, pk);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) ew[k] -= pk[k];
       }
     }
+    // a leaf has no free job (ew = tail): its makespan max_k r[k] + q[k]
+    // (criteres_calculer 897-909), or r[m-1] with an empty suffix
     const bool leaf = (l2 - l1 == 1);
     int leafv = 0;
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      leafv = max(leafv, r[k] + q[k]);
+      leafv = max(leafv, r[k] + (d2 ? ew[k] : 0));
 
-    // one-machine bound and selector features need the remaining work of
-    // every machine: head (or R0) + sum of unscheduled p + tail (or Q0)
+    // st32 = 32 x the pair start clock (heads, or R0 without prefix: initCmax
+    // 79-90).  Tables read zero past machine m, like the padded heads.
+    //
+    // Tail combine (cmaxFin 92-100, borneInfMakespan 146-148): without
+    // suffix v = t1 + Q0[b] = (U >> 5) + ew[b]; with one v = max(t1 + q[b],
+    // T0 + q[a]), and after the scan T0 = start(a) + rem(a) whatever the
+    // pair's order, so T0 + q[a] = (st32[a] >> 5) + ew[a], known when the
+    // pair starts.
+    int st32[MM];
+    {
+      int r0[MM];
+      job_row<MM>(sp, n, r0);
+#pragma unroll
+      for (int k = 0; k < MM; ++k) st32[k] = (d1 ? r[k] : r0[k]) * (1 << kScaleShift);
+    }
+    // a pair's one-machine term (machine a), or 0 without suffix
+    auto one_of = [&](int a) { return d2 ? (st32[a] >> kScaleShift) + ew[a] : 0; };
+    auto pair_value = [&](int one, int b, int U) { return max((U >> kScaleShift) + ew[b], one); };
+
+    // one-machine bound and selector features: head (or R0) + remaining
+    // work + tail (or Q0) per machine
     int lb1 = 0;
-    if (EXTRA) {
-      int rem[MM];
-#pragma unroll
-      for (int k = 0; k < MM; ++k) rem[k] = 0;
-      const int rem_hi = valid ? l2 : d1;  // padding lanes read no permutation
-      for (int i = d1; i < rem_hi; ++i) {
-        const int j = perm_at(mine, bstride, sw0, sw1, i);
-        int pk[MM];
-        job_row<MM>(sp, j, pk);
-#pragma unroll
-        for (int k = 0; k < MM; ++k)
-          rem[k] += pk[k];
-      }
+    if constexpr (EXTRA) {
 #pragma unroll
       for (int k = 0; k < MM; ++k)
         if (k < m) {
-          const int head = d1 ? r[k] : ld_const(T.R0, k);
-          const int tail = d2 ? q[k] : ld_const(T.Q0, k);
-          lb1 = max(lb1, head + rem[k] + tail);
+          const int head = st32[k] >> kScaleShift;
+          const int tail = d2 ? qx[k] : sp[(n + 1) * MM + k];
+          const int rem = ew[k] - tail;
+          lb1 = max(lb1, head + ew[k]);
           if (A.feat != nullptr && valid) {
-            const float sc = 1.0f / A.feat_scale;
-            A.feat[(0 * m + k) * ld + node] = head * sc;
-            A.feat[(1 * m + k) * ld + node] = tail * sc;
-            A.feat[(2 * m + k) * ld + node] = rem[k] * sc;
-            A.feat[(3 * m + k) * ld + node] = (head + rem[k]) * sc;
-            A.feat[(4 * m + k) * ld + node] = (rem[k] + tail) * sc;
+            const float fs = 1.0f / A.feat_scale;
+            A.feat[(0 * m + k) * ld + node] = head * fs;
+            A.feat[(1 * m + k) * ld + node] = tail * fs;
+            A.feat[(2 * m + k) * ld + node] = rem * fs;
+            A.feat[(3 * m + k) * ld + node] = (head + rem) * fs;
+            A.feat[(4 * m + k) * ld + node] = (rem + tail) * fs;
           }
         }
     }
@@ -255,61 +334,82 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
 #pragma unroll
     for (int w = 0; w < NW; ++w) um[w] = ~sm[w] & valid_bits[w];
 
-    // pair start values: heads, or R0 without prefix (initCmax 79-90)
-#pragma unroll
-    for (int k = 0; k < MM; ++k)
-      if (d1 == 0) r[k] = ld_const(T.R0, k);  // zero past machine m
-
     int lb = 0, fx = -1, am = -1;
     const bool use_best = A.best != -1;
     const uint32_t* tsel = nullptr;
     if constexpr (!LEAN || MODE == 3)
       if (A.tile_sel != nullptr) tsel = A.tile_sel + static_cast<size_t>(tile) * A.nsw;
 
-    // Pass 0 scans all pairs of [s_lo, s_hi), or the tile's learned subset;
-    // pass 1 (learned subset only) scans the complement for the nodes whose
-    // partial bound lands within `margin` below the incumbent (full LB2).
     bool take = true;
     long long fallbacks = 0;
     int npairs = 0;  // pairs scanned by this wave
-    if constexpr (LEAN) {
+    if constexpr (MODE == 0) {
+      // every pair of [s_lo, s_hi): the pairs' step groups are contiguous in
+      // the table, so they stream as one sequence, CH groups per scalar-load
+      // batch, with a pair's value taken where its groups end.  Pairs are in
+      // lexicographic order (remplirMachine 500-508), so the next pair's
+      // machines follow without a load.
       npairs = s_hi - s_lo;
-      // MODE 3 visits the set bits of the tile's mask words in order (the
-      // mask words are wave-uniform scalars); MODE 0 every pair in range
+      constexpr int GW = Rec32<NW>::GW, CH = Rec32<NW>::CH;
+      const int gpp = T.n4 >> 2;
+      const int g_lo = s_lo * gpp, g_hi = s_hi * gpp;
+      if (g_lo < g_hi) {
+        const int* gp = T.steps32 + static_cast<size_t>(g_lo) * GW;
+        const int2 ab0 = ld_const2(T.pairs, s_lo);
+        int a = ab0.x, b = ab0.y, kg = 0;
+        int D = st32[a], U = st32[b], one = one_of(a);
+        for (int g = g_lo; g < g_hi; g += CH) {
+          Chunk32<NW, CH> c;
+          c.load(gp);
+          gp += CH * GW;
+#pragma unroll
+          for (int h = 0; h < CH; ++h) {
+            if (g + h < g_hi) {
+              group32<NW>(c.w + h * GW, um, D, U);
+              if (++kg == gpp) {
+                lb = max(lb, pair_value(one, b, U));
+                kg = 0;
+                if (++b == m) {
+                  ++a;
+                  b = a + 1;
+                }
+                // past the range's last pair (a, b) may leave the machines;
+                // the clamp keeps the register index in range
+                const int ac = min(a, MM - 1);
+                D = st32[ac];
+                U = st32[min(b, MM - 1)];
+                one = one_of(ac);
+              }
+            }
+          }
+        }
+      }
+    } else if constexpr (MODE == 3) {
+      // the set bits of the tile's mask words in order (the words are
+      // wave-uniform scalars)
+      npairs = s_hi - s_lo;
       uint32_t word = 0;
       int wi = -1;
-      for (int s = s_lo;;) {
-        if constexpr (MODE == 3) {
-          while (word == 0) {
-            if (++wi >= A.nsw) break;
-            word = ld_const_u32(tsel, wi);
-          }
-          if (word == 0) break;
-          s = 32 * wi + __builtin_ctz(word);
-          word &= word - 1;
-          if (s < s_lo) continue;  // another group's share of the pairs
-          if (s >= s_hi) break;
-        } else {
-          if (s >= s_hi) break;
+      for (;;) {
+        while (word == 0) {
+          if (++wi >= A.nsw) break;
+          word = ld_const_u32(tsel, wi);
         }
+        if (word == 0) break;
+        const int s = 32 * wi + __builtin_ctz(word);
+        word &= word - 1;
+        if (s < s_lo) continue;  // another group's share of the pairs
+        if (s >= s_hi) break;
         const int2 ab = ld_const2(T.pairs, s);
-        int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
-        int t1 = r[ab.y];
-        const size_t base = static_cast<size_t>(s) * T.n4;
-        for (int k = 0; k < T.n4; k += 4) {
-          StepGroup<NW> g;
-          g.load(T, base + k);
-          step4<NW>(g, um, T0, t1);
-        }
-        // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
-        // counts (borneInfMakespan 146-148)
-        const int qa = q[ab.x], qb = q[ab.y], q0b = ld_const(T.Q0, ab.y);
-        const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + q0b;
-        lb = max(lb, v);
-        if (use_best && fx < 0 && lb > A.best) fx = lb;
-        if constexpr (MODE != 3) ++s;
+        int D = st32[ab.x], U = st32[ab.y];
+        const int one = one_of(ab.x);
+        scan_pair32<NW>(T, s, um, D, U);
+        lb = max(lb, pair_value(one, ab.y, U));
       }
     } else {
+    // Pass 0 scans all pairs of [s_lo, s_hi), or the tile's learned subset;
+    // pass 1 (learned subset only) scans the complement for the nodes whose
+    // partial bound lands within `margin` below the incumbent (full LB2).
     const int npass = (tsel != nullptr && A.margin >= 0) ? 2 : 1;
     for (int ps = 0; ps < npass; ++ps) {
       const uint32_t want = ps == 0 ? 1u : 0u;
@@ -327,18 +427,10 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
         }
         ++npairs;
         const int2 ab = ld_const2(T.pairs, s);
-        int T0 = r[ab.x] - kBias;  // machine-a clock, biased below zero
-        int t1 = r[ab.y];
-        const size_t base = static_cast<size_t>(s) * T.n4;
-        for (int k = 0; k < T.n4; k += 4) {
-          StepGroup<NW> g;
-          g.load(T, base + k);
-          step4<NW>(g, um, T0, t1);
-        }
-        // tail combine (cmaxFin 92-100); without suffix only t1 + Q0[b]
-        // counts (borneInfMakespan 146-148)
-        const int qa = q[ab.x], qb = q[ab.y];
-        const int v = d2 ? max(t1 + qb, T0 + kBias + qa) : t1 + ld_const(T.Q0, ab.y);
+        int D = st32[ab.x], U = st32[ab.y];
+        const int one = one_of(ab.x);
+        scan_pair32<NW>(T, s, um, D, U);
+        const int v = pair_value(one, ab.y, U);
         if (A.pairv != nullptr && valid) A.pairv[static_cast<size_t>(s) * ld + node] = v;
         if (take) {
           if (v > lb) {
@@ -929,7 +1021,7 @@ hipError_t launch_mm_nw(const DevTables& T, const Src& src, const BoundArgs& A,
   int grid = (ntiles + tpb - 1) / tpb;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * (((MM * T.n + 3) & ~3) + 4 * block) +
+  const size_t shmem = sizeof(int32_t) * (MM * (T.n + 3) + 4 * block) +
                        static_cast<size_t>(T.n) * 64 * tpb;
   hipExtLaunchKernelGGL((bound_kernel<MM, NW, MODE, BLOCK, Src>), dim3(grid), dim3(block), shmem,
                         stream, A.ev_start, A.ev_stop, 0, T, src, A);
@@ -943,10 +1035,12 @@ hipError_t launch_src(const DevTables& T, const Src& src, const BoundArgs& A,
   const int mm = T.m <= 8 ? 8 : (T.m <= 12 ? 12 : (T.m <= 16 ? 16 : (T.m <= 20 ? 20 : 24)));
   const bool bare = A.kind == 0 && A.argmax == nullptr && A.pairv == nullptr &&
                    A.sel == nullptr && A.stats == nullptr && A.feat == nullptr;
-  const int mode = (A.kind == 1 || A.feat != nullptr) ? 2
-                   : (bare && A.tile_sel == nullptr)               ? 0
-                   : (bare && A.margin < 0)                        ? 3
-                                                                   : 1;
+  // the `best` early exit takes the first running maximum above best in
+  // pair order, which the general mode follows pair by pair
+  const int mode = (A.kind == 1 || A.feat != nullptr)           ? 2
+                   : (bare && A.best == -1 && A.tile_sel == nullptr) ? 0
+                   : (bare && A.best == -1 && A.margin < 0)          ? 3
+                                                                     : 1;
   // 8 or 16 waves per tile (1024-thread blocks) for small batches; the lean
   // variants for up to 16 machines fit four such waves per SIMD
   if (mode == 0 && A.best == -1 && T.steps16 != nullptr && A.groups <= 4 && A.pk != 0)

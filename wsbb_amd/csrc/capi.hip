@@ -194,12 +194,31 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
         const uint32_t dab = static_cast<uint32_t>(w.y - w.z) & 0xffffu;  // p_a - p_b
         steps16[at] = uint2{static_cast<uint32_t>(w.x), dab | (static_cast<uint32_t>(K) << 16)};
       }
+  // int32 offset form: per step 32 (p_a - p_b) and 32 (lag + p_b) plus the
+  // job's bit offset (the scan's clocks carry five low bits that never reach
+  // a bound: every value below kPathLimit keeps 32 x value + 31 in 24 bits),
+  // and for n > 32 the word-select masks of the job
+  const int rs = n <= 32 ? 2 : (n <= 64 ? 3 : 4);
+  std::vector<int32_t> steps32(nsteps * rs, 0);
+  for (int s = 0; s < P; ++s)
+    for (int k = 0; k < n; ++k) {
+      const size_t at = static_cast<size_t>(s) * n4 + k;
+      const int4 w = steps[at];
+      const int j = w.x, pa = w.y, pb = w.z, lag = w.w - kBias - pb;
+      int32_t* rec = &steps32[at * rs];
+      rec[0] = (pa - pb) * (1 << kScaleShift);
+      rec[1] = (lag + pb) * (1 << kScaleShift) + (j & 31);
+      if (rs >= 3) rec[2] = static_cast<int32_t>(wsel[at].x);
+      if (rs == 4) rec[3] = static_cast<int32_t>(wsel[at].y);
+    }
   std::vector<int2> pairs(P);
   for (int s = 0; s < P; ++s) pairs[s] = int2{I.pair_a[s], I.pair_b[s]};
   const size_t o_steps = 0;
   const size_t o_wsel = o_steps + steps.size() * sizeof(int4);
-  const size_t o_s16 = o_wsel + wsel.size() * sizeof(uint2);
-  const size_t o_pairs = o_s16 + steps16.size() * sizeof(uint2);
+  auto al256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_s32 = al256(o_wsel + wsel.size() * sizeof(uint2));
+  const size_t o_s16 = al256(o_s32 + steps32.size() * sizeof(int32_t));
+  const size_t o_pairs = al256(o_s16 + steps16.size() * sizeof(uint2));
   const size_t o_p = (o_pairs + pairs.size() * sizeof(int2) + 255) & ~size_t(255);
   const size_t o_R0 = o_p + ((I.p.size() * 4 + 255) & ~size_t(255));
   const size_t o_Q0 = o_R0 + 256;
@@ -209,6 +228,7 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   std::vector<char> blob(total, 0);
   std::memcpy(blob.data() + o_steps, steps.data(), steps.size() * sizeof(int4));
   std::memcpy(blob.data() + o_wsel, wsel.data(), wsel.size() * sizeof(uint2));
+  std::memcpy(blob.data() + o_s32, steps32.data(), steps32.size() * sizeof(int32_t));
   if (two16) std::memcpy(blob.data() + o_s16, steps16.data(), steps16.size() * sizeof(uint2));
   std::memcpy(blob.data() + o_pairs, pairs.data(), pairs.size() * sizeof(int2));
   std::memcpy(blob.data() + o_p, I.p.data(), I.p.size() * 4);
@@ -224,6 +244,8 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   T.n4 = n4;
   T.steps = ctx->tables.as<const int4>(o_steps);
   T.wsel = ctx->tables.as<const uint2>(o_wsel);
+  T.steps32 = ctx->tables.as<const int>(o_s32);
+  T.rs = rs;
   T.pairs = ctx->tables.as<const int2>(o_pairs);
   T.p = ctx->tables.as<const int32_t>(o_p);
   T.R0 = ctx->tables.as<const int32_t>(o_R0);

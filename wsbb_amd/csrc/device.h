@@ -41,6 +41,11 @@ struct DevTables {
   const int32_t* R0;     // [m]
   const int32_t* Q0;     // [m]
   const int32_t* ptot;   // [m]
+  const int* steps32;    // [P][n4] int32 offset-form records of rs dwords:
+                         // {32 (p_a - p_b), 32 (lag + p_b) + (job & 31)
+                         //  [, word mask of bit 0 of job >> 5, of bit 1]};
+                         // pad = 0, plus 16 zero steps after the last pair
+  int rs;                // dwords per steps32 record: 2 (n <= 32), 3 (n <= 64), 4
   const uint2* steps16;  // [P][n4] two-node form {job, (p_a - p_b mod 2^16)
                          // | (2^14 + lag + p_b) << 16}; pad = 0, plus 16
                          // zero steps; null when bound values may reach 2^14
@@ -161,12 +166,13 @@ struct ChildSource {
   const int32_t* hr = nullptr;
   const int32_t* hq = nullptr;
   const uint32_t* hm = nullptr;
+  const int32_t* hw = nullptr;  // [MM][PN] work of the parent's scheduled jobs
   __device__ __forceinline__ bool has_state() const { return hr != nullptr; }
-  // heads, tails and scheduled mask of child k from its parent's state; sp
-  // holds the processing times job-major in LDS
+  // heads, tails, scheduled mask and scheduled work of child k from its
+  // parent's state; sp holds the processing times job-major in LDS
   template <int MM, int NW>
   __device__ __forceinline__ void state(int k, const int32_t* sp, int (&r)[MM], int (&q)[MM],
-                                        uint32_t (&sm)[NW]) const {
+                                        uint32_t (&sm)[NW], int (&sc)[MM]) const {
     const int d = desc[k];
     const int par = d >> 9, side = (d >> 8) & 1, j = d & 255;
     const int pa = pl1[par], pb = pl2[par];
@@ -174,6 +180,7 @@ struct ChildSource {
     for (int kk = 0; kk < MM; ++kk) {
       r[kk] = hr[static_cast<size_t>(kk) * PN + par];
       q[kk] = hq[static_cast<size_t>(kk) * PN + par];
+      sc[kk] = hw[static_cast<size_t>(kk) * PN + par];
     }
 #pragma unroll
     for (int w = 0; w < NW; ++w) sm[w] = hm[static_cast<size_t>(w) * PN + par];
@@ -186,6 +193,8 @@ struct ChildSource {
       head_step<MM>(r, pk);
     else
       tail_step<MM>(q, pk);
+#pragma unroll
+    for (int kk = 0; kk < MM; ++kk) sc[kk] += pk[kk];
     // the last free job goes to the end of the prefix (limites_set)
     const int a = pa + (side == 0), b = pb - (side == 1);
     if (b - a == 2) {
@@ -194,6 +203,8 @@ struct ChildSource {
       set_job_bit<NW>(sm, j2);
       job_row<MM>(sp, j2, pk);
       head_step<MM>(r, pk);
+#pragma unroll
+      for (int kk = 0; kk < MM; ++kk) sc[kk] += pk[kk];
     }
   }
   __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
@@ -225,8 +236,8 @@ struct ListSource {
   __device__ __forceinline__ bool has_state() const { return base.has_state(); }
   template <int MM, int NW>
   __device__ __forceinline__ void state(int k, const int32_t* sp, int (&r)[MM], int (&q)[MM],
-                                        uint32_t (&sm)[NW]) const {
-    base.state<MM, NW>(list[k], sp, r, q, sm);
+                                        uint32_t (&sm)[NW], int (&sc)[MM]) const {
+    base.state<MM, NW>(list[k], sp, r, q, sm, sc);
   }
   __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
                                        int& stride, int& sw0, int& sw1) const {

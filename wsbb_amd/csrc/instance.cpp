@@ -146,7 +146,9 @@ std::string Instance::build(const int32_t* p_mm, int n_, int m_) {
     for (int j = 0; j < n; ++j) mx = std::max(mx, p[k * n + j]);
     pb += mx;
   }
-  if (pb >= kBias / 2) return "instance too large for the 24-bit bound arithmetic";
+  if (pb >= kPathLimit)
+    return "instance too large for the 24-bit bound arithmetic: sum over jobs of the largest "
+           "time plus sum over machines of the largest time must stay below 2^19";
   path_bound = static_cast<int>(pb);
 
   pair_a.resize(P);

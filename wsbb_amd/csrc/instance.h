@@ -13,6 +13,11 @@ constexpr int kBias = 1 << 22;
 // The same offset for the two-node int16 form, used when every bound value
 // of the instance stays below it.
 constexpr int kBias16 = 1 << 14;
+// The int32 scan keeps its clocks scaled by 32 (kScaleShift bits) with the
+// job's bit offset in the low bits of a 24-bit multiply-add operand: every
+// step constant and bound value must stay below this limit.
+constexpr int kScaleShift = 5;
+constexpr int kPathLimit = 1 << (24 - kScaleShift);
 constexpr int kMaxJobs = 128;     // 4 x 32-bit job masks per node
 constexpr int kMaxMachines = 24;  // register arrays for heads/tails
 

@@ -101,6 +101,7 @@ struct SArgs {
   int* p_hr;        // [MM][PCAP + 1] heads of the parent's prefix
   int* p_hq;        // [MM][PCAP + 1] tails of the parent's suffix
   uint32_t* p_hm;   // [NW][PCAP + 1] the parent's scheduled jobs
+  int* p_hw;        // [MM][PCAP + 1] work of the parent's scheduled jobs
   // selector features of the parents written by expand (m a multiple of
   // four): BF16 MFMA fragments and/or FP32 rows [5m][x_ld], or null
   uint2* x_frag;
@@ -545,6 +546,7 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, i
       for (int k = 0; k < MM; ++k) {
         S.p_hr[k * PH + p] = r[k];
         S.p_hq[k * PH + p] = q[k];
+        S.p_hw[k * PH + p] = wk[k];
       }
 #pragma unroll
       for (int w = 0; w < NW; ++w) S.p_hm[w * PH + p] = sm[w];
@@ -1123,6 +1125,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_hr = take(4 * size_t(mmp) * (PCAP + 1));
   const size_t o_hq = take(4 * size_t(mmp) * (PCAP + 1));
   const size_t o_hm = take(4 * size_t(nwp) * (PCAP + 1));
+  const size_t o_hw = take(4 * size_t(mmp) * (PCAP + 1));
   hipError_t e = S->mem.reserve(off);
   if (e != hipSuccess) {
     destroy_search(ctx);
@@ -1158,6 +1161,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.p_hr = S->mem.as<int>(o_hr);
   A.p_hq = S->mem.as<int>(o_hq);
   A.p_hm = S->mem.as<uint32_t>(o_hm);
+  A.p_hw = S->mem.as<int>(o_hw);
   return WSBB_OK;
 }
 
@@ -1194,6 +1198,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     src.hr = A.p_hr;
     src.hq = A.p_hq;
     src.hm = A.p_hm;
+    src.hw = A.p_hw;
   }
   SearchState* SS = ctx->search;
   const int log_stride = selector_log_stride(ctx);
