@@ -1,25 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark: LB2 bounded nodes/s and time-to-proven-optimum on one MI355X.
 
-Default workload (one step = one solve): Taillard ta011, 20 jobs x 10
-machines, int32 times in [1, 99], solved to proven optimum from an infinite
-upper bound with the device-resident B&B using plain LB2, pools of 4096
-children per sub-batch and ten sub-batches per round.  `value` is children
-bounded per second over the K timed solves; ms_per_step is the
-time-to-optimum.
+Headline (one step = one bound launch): Taillard ta051, 50 jobs x 20
+machines, int32 times in [1, 99]; a batch of 2^20 synthetic nodes (prefix
+and suffix lengths uniform so the scheduled depth is in [0, 49], random job
+permutations) resident in HBM, bounded with plain LB2 by the int32 kernel
+with autotuned launch shape.  `value` is nodes bounded per second over the K
+timed launches; `roofline` relates the algorithmic integer work of the batch
+to the kernel's launch time (HIP events on the context stream) and the int32
+VALU peak.
 
-Also reported ("batched"): the 50x20 batched bounding throughput, 2^20
-synthetic nodes (prefix/suffix lengths uniform, random permutations) per
-launch, inputs resident in HBM; ("learned_20x20"): plain against learned
-pair selection on ta021 solved to optimum, the selector trained on a
-plain solve of the same instance; ("learned_online"): ta021-ta024, the
-selector trained during the search on records of its first 2e6 bounded
-children, every cost counted; ("mixed_batch"): 64 instances of
-uneven size bounded together, plain and learned, over 1e8 nodes.
-
-The CPU baseline is the oracle (the C restatement of the reference CPU path)
-on this host: the same ta011 solve with the reference's DFS explorer, one
-thread, and a 2^15-node sample of the batched workload.
+Sub-objects: "int16_two_node" (the same batch by the two-node int16 kernel,
+against the packed 16-bit issue rate), "config1" (ta001-ta010, 20x5, solved
+with LB2 and LB1 on the GPU and by the CPU explorer), "config2" (ta011, 20x10,
+time-to-optimum with 4096-child pools: ten per round, and one per round as
+the reference launches them), "learned_20x20" and "learned_online" (20x20
+plain against learned pair selection), "mixed_batch" (64 instances of uneven
+size in one launch).  CPU baselines are the oracle (the C restatement of the
+reference CPU path) on this host, on one thread and on all host cores.
 """
 import argparse
 import json
@@ -61,57 +59,166 @@ def node_work(n, m, l1, l2):
 TIME_EVERY = 8  # the bound launch of every 8th round carries HIP events
 
 
-def bench_solve(wsbb_amd, steps, warmup):
-    p = wsbb_amd.taillard(11)
+def host_info():
+    """CPU of this host: logical CPUs, usable CPUs, model and clock."""
+    info = {"nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0))}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and "model" not in info:
+                    info["model"] = line.split(":", 1)[1].strip()
+                if line.startswith("cpu MHz") and "mhz" not in info:
+                    info["mhz"] = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    # the GPU box allots 16 host threads per GPU
+    info["threads"] = max(1, min(info["usable"], 16))
+    return info
+
+
+def bench_solve(wsbb_amd, p, reps, pool=4096, nb=10, kind=0, profile=True):
+    """`reps` solves of instance p to proven optimum after one untimed one;
+    optional bound-kernel timing on every 8th round."""
     eng = wsbb_amd.Engine(p)
-    eng.search_profile(TIME_EVERY)
-    for _ in range(warmup):
-        eng.solve(pool_size=4096, max_batches=10)
+    if profile:
+        eng.search_profile(TIME_EVERY)
+    res = eng.solve(kind=kind, pool_size=pool, max_batches=nb)
     eng.synchronize()
     tot = dict(bounded=0, work=0, bound_ms=0.0, launches=0, rounds=0)
-    best = None
     t0 = time.perf_counter()
-    for _ in range(steps):
-        res = eng.solve(pool_size=4096, max_batches=10)
-        ms, launches = eng.search_profile(-1)
+    for _ in range(reps):
+        res = eng.solve(kind=kind, pool_size=pool, max_batches=nb)
+        assert res["proven"]
+        if profile:
+            ms, launches = eng.search_profile(-1)
+            tot["bound_ms"] += ms
+            tot["launches"] += launches
         tot["bounded"] += res["stats"]["bounded"]
         tot["work"] += res["stats"]["work_ops"]
-        tot["bound_ms"] += ms
-        tot["launches"] += launches
         tot["rounds"] += res["stats"]["rounds"]
-        best = res["best"]
-        assert res["proven"]
     eng.synchronize()
-    t1 = time.perf_counter()
-    stats = res["stats"]
+    elapsed = time.perf_counter() - t0
     eng.close()
-    return dict(elapsed=t1 - t0, best=best, stats=stats, **tot)
+    return dict(elapsed=elapsed, seconds=elapsed / reps, best=res["best"], stats=res["stats"],
+                **tot)
 
 
-def bench_batched(wsbb_amd, reps=20, autotune=True):
+def cpu_solves(o, kind, host):
+    """The oracle's explorer on one thread and on the host's threads."""
+    import oracle
+    t0 = time.perf_counter()
+    b1, s1, _ = o.solve_dfs(kind=kind)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bn, sn, _ = o.solve_parallel(kind=kind, threads=host["threads"])
+    tn = time.perf_counter() - t0
+    assert b1 == bn
+    return {"optimum": b1, "one_thread": {"seconds": t1, "bounded": s1["bounded"]},
+            "all_threads": {"seconds": tn, "bounded": sn["bounded"], "threads": host["threads"]}}
+
+
+def bench_config1(wsbb_amd, host, cpu=True):
+    """ta001-ta010 (20x5) solved to proven optimum with LB2 and with LB1:
+    the GPU search with 4096-child pools (one per round) against the CPU
+    explorer (DFS, begin-end branching, UB = +inf)."""
+    import oracle
+    rows = []
+    for tid in range(1, 11):
+        p = wsbb_amd.taillard(tid)
+        row = {"instance": f"ta{tid:03d}"}
+        for name, kind, okind in (("lb2", wsbb_amd.LB2, oracle.LB2),
+                                  ("lb1", wsbb_amd.LB1, oracle.LB1)):
+            g = bench_solve(wsbb_amd, p, 3, pool=4096, nb=1, kind=kind, profile=False)
+            ent = {"gpu": {"optimum": g["best"], "seconds": g["seconds"],
+                           "bounded": g["stats"]["bounded"], "rounds": g["stats"]["rounds"]}}
+            if cpu:
+                c = cpu_solves(oracle.Oracle(p), okind, host)
+                ent["cpu"] = c
+                ent["optimum_match"] = c["optimum"] == g["best"]
+            row[name] = ent
+        rows.append(row)
+    tot = {}
+    for name in ("lb2", "lb1"):
+        tot[name] = {"gpu_seconds": sum(r[name]["gpu"]["seconds"] for r in rows),
+                     "gpu_bounded": sum(r[name]["gpu"]["bounded"] for r in rows)}
+        if cpu:
+            tot[name]["cpu_one_thread_seconds"] = sum(r[name]["cpu"]["one_thread"]["seconds"]
+                                                      for r in rows)
+            tot[name]["cpu_all_threads_seconds"] = sum(r[name]["cpu"]["all_threads"]["seconds"]
+                                                       for r in rows)
+            tot[name]["all_optima_match"] = all(r[name]["optimum_match"] for r in rows)
+    return {"workload": "ta001-ta010 20x5 solved to proven optimum, LB2 and LB1, UB=+inf; "
+                        "GPU: 4096-child pools, one per round; CPU: the oracle's DFS explorer",
+            "totals": tot, "instances": rows}
+
+
+def bench_config2(wsbb_amd, host, reps=10, cpu=True):
+    """ta011 (20x10) to proven optimum: pools of 4096 children, ten per round
+    (the reference's ring of ten pools in flight) and one per round (one
+    pool per launch); roofline of the bound kernel inside the solve."""
+    import oracle
+    p = wsbb_amd.taillard(11)
+    ten = bench_solve(wsbb_amd, p, reps, pool=4096, nb=10)
+    one = bench_solve(wsbb_amd, p, max(1, reps // 2), pool=4096, nb=1)
+    out = {"workload": "ta011 20x10 solved to proven optimum, plain LB2, 4096-child pools, "
+                       "UB=+inf"}
+    for name, sv in (("ten_pools_per_round", ten), ("one_pool_per_round", one)):
+        avg_ms = sv["bound_ms"] / max(sv["launches"], 1)
+        bound_s = avg_ms * sv["rounds"] / 1e3
+        ach = sv["work"] / bound_s / 1e12 if bound_s > 0 else None
+        out[name] = {
+            "time_to_optimum_s": sv["seconds"], "optimum": sv["best"],
+            "nodes_per_s": sv["bounded"] / sv["elapsed"],
+            "nodes_bounded_per_solve": sv["stats"]["bounded"],
+            "rounds_per_solve": sv["stats"]["rounds"],
+            "bound_kernel": {"avg_launch_us": avg_ms * 1e3, "share_of_solve": bound_s / sv["elapsed"],
+                             "achieved": ach, "peak": PEAK_VALU_TOPS, "unit": "Tint-op/s",
+                             "frac": ach / PEAK_VALU_TOPS if ach else None,
+                             "timed_launches": sv["launches"]},
+        }
+    if cpu:
+        c = cpu_solves(oracle.Oracle(p), oracle.LB2, host)
+        c["note"] = ("the CPU explorer pops one node at a time (DFS) and bounds fewer "
+                     "children than the batched GPU order")
+        out["cpu"] = c
+        out["optimum_match"] = c["optimum"] == ten["best"] == one["best"]
+    return out
+
+
+def bench_batched(wsbb_amd, steps=20, warmup=2, int16=False, autotune=True):
+    """2^20 ta051 nodes bounded `steps` times by one launch each, after
+    `warmup` untimed launches.  Returns the wall time and the HIP-event time
+    (context stream) per launch and the batch's algorithmic work."""
     p = wsbb_amd.taillard(51)
     m, n = p.shape
     N = 1 << 20
     permT, l1, l2 = wsbb_amd.random_nodes(n, N, 0x5EED)
     eng = wsbb_amd.Engine(p)
+    eng.set_int16(2 if int16 else 0)
     tuned = eng.autotune(N, reps=3) if autotune else []
     dp, d1, d2, do = eng.malloc(permT.nbytes), eng.malloc(4 * N), eng.malloc(4 * N), eng.malloc(4 * N)
     eng.h2d(dp, permT)
     eng.h2d(d1, l1)
     eng.h2d(d2, l2)
-    eng.bound_device(dp, d1, d2, N, do)
-    eng.synchronize()
-    eng.timer_start()
-    for _ in range(reps):
+    for _ in range(max(warmup, 1)):
         eng.bound_device(dp, d1, d2, N, do)
-    ms = eng.timer_stop() / reps
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.timer_start()
+    for _ in range(steps):
+        eng.bound_device(dp, d1, d2, N, do)
+    ev_ms = eng.timer_stop() / steps
+    eng.synchronize()
+    wall_ms = (time.perf_counter() - t0) / steps * 1e3
+    out = np.zeros(N, np.int32)
+    eng.d2h(out, do)
     for d in (dp, d1, d2, do):
         eng.free(d)
     eng.close()
     work = int(node_work(n, m, l1, l2).sum())
     bytes_alg = N * (n + 12)  # uint8 permutation + two int32 limits in, int32 bound out
-    return dict(N=N, ms=ms, work=work, bytes=bytes_alg, tuned=tuned, permT=permT, l1=l1, l2=l2,
-                p=p)
+    return dict(N=N, ms=ev_ms, wall_ms=wall_ms, work=work, bytes=bytes_alg, tuned=tuned,
+                permT=permT, l1=l1, l2=l2, p=p, out=out)
 
 
 def children_of(permT, l1, l2):
@@ -198,59 +305,75 @@ def bench_learned(wsbb_amd):
     }
 
 
-def bench_learned_online(wsbb_amd, ids=(21, 22, 23, 24), warm=2_000_000, topk=24):
-    """Online learning on one instance, all costs counted: a plain solve that
-    logs (parent features, argmax pair) records stops after `warm` bounded
-    children, the search is checkpointed, the selector is trained from
-    scratch on those records (2 epochs, 2^17 records at most), and the search
-    resumes with learned pair selection (top-`topk` pairs per tile, no
-    fallback) to the proven optimum.  The time of the whole sequence is set
-    against a plain solve of the same instance from scratch."""
+def online_run(wsbb_amd, eng, pool, nb, warm, cap, topk):
+    """One online-learning solve on a context: a plain solve that logs
+    records stops after `warm` bounded children, the search is checkpointed,
+    the selector is trained from scratch on the records and the search
+    resumes with learned selection.  Returns (result, seconds, training
+    seconds, bounded children)."""
+    eng.selector_init(seed=1)
+    eng.selector_config(topk=topk, margin=-1, parents=True)
+    eng.selector_log(stride=max(1, warm * eng.n // cap), capacity=cap)
+    t = time.perf_counter()
+    first = eng.solve(pool_size=pool, max_batches=nb, budget=warm)
+    bounded = first["stats"]["bounded"]
+    train_s = 0.0
+    if first["proven"]:
+        res = first
+    else:
+        ck = eng.checkpoint()
+        t2 = time.perf_counter()
+        eng.selector_train(epochs=2, batch=2048, lr=2e-3, seed=2)
+        train_s = time.perf_counter() - t2
+        eng.selector_log(stride=0, capacity=0)
+        res = eng.resume(ck, kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
+        bounded = res["stats"]["bounded"]
+    return res, time.perf_counter() - t, train_s, bounded
+
+
+def bench_learned_online(wsbb_amd, ids=tuple(range(21, 31)), runs=3, warm=2_000_000, topk=24):
+    """Online learning on one instance, all costs counted (online_run), set
+    against a plain solve of the same instance from scratch; `runs` of each
+    per instance, alternating, with mean and spread of the times."""
     pool, nb, cap = 65536, 4, 1 << 17
     out = []
     for tid in ids:
+        plain_t, online_t, train_t = [], [], []
         with wsbb_amd.Engine(wsbb_amd.taillard(tid)) as eng:
             eng.solve(pool_size=pool, max_batches=nb, budget=20_000)  # warm up
-            t = time.perf_counter()
-            plain = eng.solve(pool_size=pool, max_batches=nb)
-            plain_s = time.perf_counter() - t
-            eng.selector_init(seed=1)
-            eng.selector_config(topk=topk, margin=-1, parents=True)
-            eng.selector_log(stride=max(1, warm * eng.n // cap), capacity=cap)
-            t = time.perf_counter()
-            first = eng.solve(pool_size=pool, max_batches=nb, budget=warm)
-            bounded = first["stats"]["bounded"]
-            train_s = 0.0
-            if first["proven"]:
-                res = first
-            else:
-                ck = eng.checkpoint()
-                t2 = time.perf_counter()
-                eng.selector_train(epochs=2, batch=2048, lr=2e-3, seed=2)
-                train_s = time.perf_counter() - t2
-                eng.selector_log(stride=0, capacity=0)
-                res = eng.resume(ck, kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
-                bounded = res["stats"]["bounded"]
-            online_s = time.perf_counter() - t
+            for _ in range(runs):
+                t = time.perf_counter()
+                plain = eng.solve(pool_size=pool, max_batches=nb)
+                plain_t.append(time.perf_counter() - t)
+                res, sec, tr, bounded = online_run(wsbb_amd, eng, pool, nb, warm, cap, topk)
+                online_t.append(sec)
+                train_t.append(tr)
+                assert res["proven"] and res["best"] == plain["best"]
+        pm, om = float(np.mean(plain_t)), float(np.mean(online_t))
         out.append({
-            "instance": f"ta{tid:03d}",
-            "plain": {"optimum": plain["best"], "proven": plain["proven"], "seconds": plain_s,
-                      "bounded": plain["stats"]["bounded"]},
-            "online": {"optimum": res["best"], "proven": res["proven"], "seconds": online_s,
-                       "train_seconds": train_s, "bounded": bounded},
-            "speedup_time_to_optimum": plain_s / online_s,
+            "instance": f"ta{tid:03d}", "optimum": plain["best"],
+            "plain_s": {"mean": pm, "min": min(plain_t), "max": max(plain_t)},
+            "online_s": {"mean": om, "min": min(online_t), "max": max(online_t)},
+            "train_s_mean": float(np.mean(train_t)),
+            "plain_bounded": plain["stats"]["bounded"], "online_bounded": bounded,
+            "speedup_mean": pm / om,
+            "speedup_range": [min(plain_t) / max(online_t), max(plain_t) / min(online_t)],
             "node_inflation": bounded / plain["stats"]["bounded"],
         })
+    sp = [r["speedup_mean"] for r in out]
     return {
         "protocol": f"plain solve with record logging for the first {warm} bounded children, "
                     "checkpoint, selector trained from scratch on the GPU (2 epochs, at most "
                     f"{cap} records, batch 2048), learned resume (top-{topk} pairs per 64-child "
-                    "tile, no fallback) to the proven optimum; all of it timed",
+                    "tile, no fallback) to the proven optimum; all of it timed, against a plain "
+                    f"solve from scratch; {runs} runs of each per instance",
+        "geomean_speedup": float(np.exp(np.mean(np.log(sp)))),
+        "instances_faster": int(sum(x > 1.0 for x in sp)),
         "instances": out,
     }
 
 
-def bench_mixed(wsbb_amd, budget=100_000_000):
+def bench_mixed(wsbb_amd, budget=100_000_000, host=None):
     """64 instances, n in {20, 50, 100} x m in {5, 10, 20}, job-correlated
     times, 16384 random nodes each, bounded together in one launch per pass
     until `budget` nodes are bounded: plain LB2 and learned pair selection
@@ -301,7 +424,32 @@ def bench_mixed(wsbb_amd, budget=100_000_000):
         reps = -(-budget // N)
         full, st_p, ms_p = mb.run(kind=wsbb_amd.LB2, reps=reps)
         sub, st_l, ms_l = mb.run(kind=wsbb_amd.LEARNED, inc=inc, margin=30, reps=reps)
+    cpu = None
+    if host is not None:
+        import oracle
+        # the first 256 nodes of every instance, once on one thread and once
+        # (the first 2048) on the host's threads
+        t1 = tn = 0.0
+        for i, p in enumerate(ps):
+            o = oracle.Oracle(p)
+            n = p.shape[1]
+            sl = slice(i * per, i * per + 256)
+            t = time.perf_counter()
+            ref = o.bound_batch(permT[:n, sl], l1[sl], l2[sl], threads=1)
+            t1 += time.perf_counter() - t
+            assert np.array_equal(ref, full[sl])
+            sl = slice(i * per, i * per + 2048)
+            t = time.perf_counter()
+            o.bound_batch(permT[:n, sl], l1[sl], l2[sl], threads=host["threads"])
+            tn += time.perf_counter() - t
+        cpu = {"one_thread": {"value": 64 * 256 / t1, "unit": "nodes/s", "cores": 1,
+                              "sample": "first 256 nodes of each instance, oracle LB2"},
+               "all_threads": {"value": 64 * 2048 / tn, "unit": "nodes/s",
+                               "cores": host["threads"],
+                               "sample": "first 2048 nodes of each instance, oracle LB2"},
+               "kind": "port", "parity": "GPU plain bounds equal the oracle on the sample"}
     return {
+        "cpu_baseline": cpu,
         "workload": f"64 instances (n x m over {{20,50,100}} x {{5,10,20}}, p_ij = a_j + U[0,20], "
                     f"a_j ~ U[1,80]), {per} random nodes each, {N} per launch, {reps} passes "
                     f"= {reps * N} nodes",
@@ -318,41 +466,52 @@ def bench_mixed(wsbb_amd, budget=100_000_000):
     }
 
 
-def cpu_baselines(bt):
+def cpu_batched(bt, host):
+    """The oracle's LB2 on the head of the same batch: 2^17 nodes on one
+    thread, 2^19 on the host's threads; the GPU's values are checked on the
+    one-thread sample."""
     import oracle
-    p = oracle.taillard(11)[3]
-    o = oracle.Oracle(p)
-    t0 = time.perf_counter()
-    best, st, _ = o.solve_dfs()
-    dt = time.perf_counter() - t0
-    solve = dict(value=st["bounded"] / dt, seconds=dt, bounded=st["bounded"], best=best)
     ob = oracle.Oracle(bt["p"])
-    S = 1 << 15
+    S1, S2 = 1 << 17, 1 << 19
     t0 = time.perf_counter()
-    ref = ob.bound_batch(bt["permT"][:, :S], bt["l1"][:S], bt["l2"][:S], threads=1)
-    dtb = time.perf_counter() - t0
-    cores = min(os.cpu_count() or 1, 16)
-    S2 = 1 << 17
+    ref = ob.bound_batch(bt["permT"][:, :S1], bt["l1"][:S1], bt["l2"][:S1], threads=1)
+    t1 = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ob.bound_batch(bt["permT"][:, :S2], bt["l1"][:S2], bt["l2"][:S2], threads=cores)
-    dtc = time.perf_counter() - t0
-    return solve, dict(value=S / dtb, seconds=dtb, sample=S, ref=ref,
-                       all_cores=dict(value=S2 / dtc, cores=cores, sample=S2))
+    ob.bound_batch(bt["permT"][:, :S2], bt["l1"][:S2], bt["l2"][:S2], threads=host["threads"])
+    tn = time.perf_counter() - t0
+    return {"value": S2 / tn, "unit": "nodes/s", "cores": host["threads"], "kind": "port",
+            "sample": f"first {S2} nodes of the same batch, oracle LB2, {host['threads']} threads "
+                      f"({tn:.2f} s)",
+            "one_thread": {"value": S1 / t1, "unit": "nodes/s", "cores": 1,
+                           "sample": f"first {S1} nodes, 1 thread ({t1:.2f} s)"},
+            "host": {k: host.get(k) for k in ("model", "mhz", "nproc", "usable")},
+            "parity": bool(np.array_equal(ref, bt["out"][:S1]))}
+
+
+def roofline(bt, peak, note):
+    ach = bt["work"] / (bt["ms"] / 1e3) / 1e12
+    return {"bound": "valu", "kernel": note, "achieved": ach, "peak": peak, "unit": "Tint-op/s",
+            "frac": ach / peak, "avg_launch_us": bt["ms"] * 1e3,
+            "hbm_gbs": bt["bytes"] / (bt["ms"] / 1e3) / 1e9,
+            "hbm_frac": bt["bytes"] / (bt["ms"] / 1e3) / 1e9 / PEAK_HBM_GBS,
+            "algorithmic_bytes": bt["bytes"], "algorithmic_ops": bt["work"]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("all", "solve", "batched", "copy", "learned"),
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned"),
                     default="all",
-                    help="solve = ta011 time-to-optimum only; batched = 50x20 launches only; "
-                         "learned = the ta021 plain-against-learned comparison only")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+                    help="batched = the 50x20 launches only (--int16: two-node form); solve = "
+                         "ta011 time-to-optimum only; learned = the 20x20 learned comparison")
+    ap.add_argument("--int16", action="store_true", help="batched: the two-node int16 kernel")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the 20x20 learned-selector and mixed-batch measurements")
+    ap.add_argument("--online-runs", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build targets a single MI355X "
@@ -360,6 +519,7 @@ def main():
         sys.exit(2)
 
     import wsbb_amd
+    host = host_info()
     if args.workload == "copy":  # calibration of the hardware byte counters
         eng = wsbb_amd.Engine(wsbb_amd.taillard(1))
         nbytes = 256 << 20
@@ -375,93 +535,61 @@ def main():
         return
     if args.workload == "learned":
         print(json.dumps({"learned_20x20": bench_learned(wsbb_amd),
-                          "learned_online": bench_learned_online(wsbb_amd)}), flush=True)
+                          "learned_online": bench_learned_online(wsbb_amd,
+                                                                 runs=args.online_runs)}),
+              flush=True)
         return
+    if args.workload == "solve":
+        print(json.dumps(bench_config2(wsbb_amd, host, reps=args.steps, cpu=not args.no_cpu)),
+              flush=True)
+        return
+    bt = bench_batched(wsbb_amd, args.steps, args.warmup, int16=args.int16 and
+                       args.workload == "batched", autotune=not args.no_autotune)
     if args.workload == "batched":
-        bt = bench_batched(wsbb_amd, reps=args.steps, autotune=not args.no_autotune)
-        print(json.dumps({"workload": "batched", "ms_per_launch": bt["ms"],
+        print(json.dumps({"workload": "batched", "int16": args.int16, "ms_per_launch": bt["ms"],
                           "nodes_per_s": bt["N"] / (bt["ms"] / 1e3)}), flush=True)
         return
-    sv = bench_solve(wsbb_amd, args.steps, args.warmup)
-    bt = None if args.workload == "solve" else bench_batched(
-        wsbb_amd, autotune=not args.no_autotune)
-    cpu_solve = cpu_b = None
-    if not args.no_cpu and bt:
-        cpu_solve, cpu_b = cpu_baselines(bt)
 
-    value = sv["bounded"] / sv["elapsed"]
-    # average bound-launch time from the timed rounds, times every round
-    avg_ms = sv["bound_ms"] / max(sv["launches"], 1)
-    bound_s = avg_ms * sv["rounds"] / 1e3
-    ach = sv["work"] / bound_s / 1e12 if bound_s > 0 else None
+    rf = roofline(bt, PEAK_VALU_TOPS, "bound_kernel<20, 2, 0, 256, BatchSource> (int32)")
+    rf["traffic"] = measured_traffic("batched_int32")
     line = {
         "metric": "LB2 bounded nodes/s + time-to-proven-optimum, 1 MI355X vs host CPU",
-        "value": value,
+        "value": bt["N"] / (bt["wall_ms"] / 1e3),
         "unit": "nodes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sv["elapsed"] / args.steps * 1e3,
+        "ms_per_step": bt["wall_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (Taillard generator, seeds of the reference)",
-        "config": {"workload": "ta011 20x10 solved to proven optimum, plain LB2, 4096-child "
-                               "pools, 10 sub-batches per round, UB=+inf",
+        "data": "synthetic (Taillard generator with the reference's seeds; splitmix64 nodes)",
+        "config": {"workload": "ta051 50x20 batched LB2 bounding: 2^20 nodes per launch, "
+                               "d1~U{0..49}, d2~U{0..49-d1}, random permutations, resident "
+                               "in HBM; int32 kernel, autotuned launch shape",
                    "parallelism": "single GPU"},
-        "time_to_optimum_s": sv["elapsed"] / args.steps,
-        "optimum": sv["best"],
-        "nodes_bounded_per_solve": sv["stats"]["bounded"],
-        "rounds_per_solve": sv["stats"]["rounds"],
-        "roofline": {"bound": "valu", "kernel": "bound_kernel (children of a round)",
-                     "achieved": ach, "peak": PEAK_VALU_TOPS, "unit": "Tint-op/s",
-                     "frac": (ach / PEAK_VALU_TOPS) if ach else None,
-                     "traffic": measured_traffic("solve"),
-                     "algorithmic_bytes": sv["stats"]["bounded"] / max(sv["stats"]["rounds"], 1)
-                     * (20 + 12),
-                     "avg_launch_us": avg_ms * 1e3,
-                     "launches": sv["rounds"],
-                     "timed_launches": sv["launches"],
-                     "timing": f"HIP events on the bound launch of every {TIME_EVERY}th round "
-                               "inside the timed solves (the launch records them itself)",
-                     "share_of_solve": bound_s / sv["elapsed"]},
+        "roofline": rf,
+        "autotune": bt["tuned"],
     }
-    if cpu_solve:
-        line["cpu_baseline"] = {"value": cpu_solve["value"], "unit": "nodes/s", "cores": 1,
-                                "kind": "port",
-                                "sample": "ta011 solved to proven optimum by the oracle's DFS "
-                                          "explorer (reference CPU path), 1 thread",
-                                "time_to_optimum_s": cpu_solve["seconds"],
-                                "optimum": cpu_solve["best"]}
-    if bt:
-        ach_b = bt["work"] / (bt["ms"] / 1e3) / 1e12
-        line["batched"] = {
-            "workload": "ta051 50x20, 2^20 random nodes per launch, d1~U{0..49}, "
-                        "d2~U{0..49-d1}, inputs resident in HBM",
-            "nodes_per_s": bt["N"] / (bt["ms"] / 1e3),
-            "ms_per_launch": bt["ms"],
-            "roofline": {"bound": "valu", "achieved": ach_b, "peak": PEAK_VALU_TOPS,
-                         "unit": "Tint-op/s", "frac": ach_b / PEAK_VALU_TOPS,
-                         "hbm_gbs": bt["bytes"] / (bt["ms"] / 1e3) / 1e9,
-                         "hbm_frac": bt["bytes"] / (bt["ms"] / 1e3) / 1e9 / PEAK_HBM_GBS,
-                         "traffic": measured_traffic("batched"),
-                         "algorithmic_bytes": bt["bytes"]},
-            "autotune": bt["tuned"],
-        }
-        if cpu_b:
-            line["batched"]["cpu_baseline"] = {
-                "value": cpu_b["value"], "unit": "nodes/s", "cores": 1, "kind": "port",
-                "sample": f"first {cpu_b['sample']} nodes of the same batch, oracle LB2, 1 thread"}
-            ac = cpu_b["all_cores"]
-            line["batched"]["cpu_baseline_all_cores"] = {
-                "value": ac["value"], "unit": "nodes/s", "cores": ac["cores"], "kind": "port",
-                "sample": f"first {ac['sample']} nodes of the same batch, oracle LB2, "
-                          f"{ac['cores']} threads"}
-    if args.workload == "all" and not args.no_learned:
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_batched(bt, host)
+    b16 = bench_batched(wsbb_amd, args.steps, args.warmup, int16=True,
+                        autotune=not args.no_autotune)
+    r16 = roofline(b16, 2 * PEAK_VALU_TOPS, "bound_kernel_pk<20, 2, BatchSource> (two nodes "
+                   "per lane in int16 halves; peak = packed 16-bit issue rate, 2 x int32)")
+    r16["traffic"] = measured_traffic("batched_int16")
+    line["int16_two_node"] = {
+        "dtype": "int16x2", "nodes_per_s": b16["N"] / (b16["wall_ms"] / 1e3),
+        "ms_per_launch": b16["wall_ms"], "roofline": r16,
+        "same_values_as_int32": bool(np.array_equal(b16["out"], bt["out"])),
+        "applies": "instances whose path bound stays below 2^14 (every Taillard instance)"}
+    line["config1"] = bench_config1(wsbb_amd, host, cpu=not args.no_cpu)
+    line["config2"] = bench_config2(wsbb_amd, host, cpu=not args.no_cpu)
+    if not args.no_learned:
         line["learned_20x20"] = bench_learned(wsbb_amd)
-        line["learned_online"] = bench_learned_online(wsbb_amd)
-        line["mixed_batch"] = bench_mixed(wsbb_amd)
+        line["learned_online"] = bench_learned_online(wsbb_amd, runs=args.online_runs)
+        line["mixed_batch"] = bench_mixed(wsbb_amd, host=None if args.no_cpu else host)
     print(json.dumps(line), flush=True)
 
 

@@ -65,6 +65,8 @@ def lib():
         L.orc_solve_batched_ex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                            ctypes.c_int, _i64p, _i32p]
+        L.orc_solve_parallel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _i64p, _i32p]
         L.orc_trace_begin.argtypes = [_i32p, ctypes.c_int64, ctypes.c_int]
         L.orc_trace_end.restype = ctypes.c_int64
         _lib = L
@@ -209,6 +211,14 @@ class Oracle:
             self.h, kind, order, ub0, budget, _p(st, _i64p), _p(bp)), trace)
         res = (best, dict(zip(STAT_NAMES, st.tolist())), bp)
         return res + (tr,) if trace else res
+
+    def solve_parallel(self, kind=LB2, threads=8, ub0=2**31 - 1):
+        """All-cores CPU explorer over independent subtrees (same optimum
+        as solve_dfs; counters depend on thread timing)."""
+        st = np.zeros(NSTATS, np.int64)
+        bp = np.zeros(self.n, np.int32)
+        best = lib().orc_solve_parallel(self.h, kind, threads, ub0, _p(st, _i64p), _p(bp))
+        return best, dict(zip(STAT_NAMES, st.tolist())), bp
 
     def solve_batched(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0,
                       order=1, trace=0):

@@ -1023,3 +1023,203 @@ int orc_solve_batched_ex(const orc_inst *h, int kind, int pool_size,
   if (stats) memcpy(stats, st, sizeof(st));
   return exhausted ? -1 : best;
 }
+
+/* ------------------------------------------------------------------------ */
+/* All-cores CPU search (CPU baseline only): the CPU explorer's rules        */
+/* (explorer.cu:92-148) run by nthreads threads over independent subtrees.   */
+/* The top of the tree is expanded sequentially, level by level, until the  */
+/* frontier holds at least 64 nodes per thread; the frontier's nodes are     */
+/* then taken one at a time (top of the stack first) by the threads, each    */
+/* running the DFS explorer on its subtree with one shared incumbent (read   */
+/* at every pop, lowered under a lock by strictly better leaves).  A thread  */
+/* that finds the shared stack empty raises a flag, and a busy thread then   */
+/* gives it the bottom (shallowest) node of its own stack.  The optimum is   */
+/* the sequential one; node counts depend on when incumbents are shared.     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  const orc_inst *h;
+  int kind, nthreads;
+  pool_t *roots; /* shared subtrees (stack), taken and given under lock */
+  pthread_mutex_t lock;
+  pthread_cond_t cond;
+  int idle, done;
+  volatile int hungry; /* an idle thread waits for work */
+  volatile int best;
+  int have;
+  int *best_perm;
+  int64_t st[ORC_NSTATS];
+} par_ctx;
+
+/* the bottom node of a stack pool (its shallowest: the largest subtree) */
+static void pool_take_bottom(pool_t *pl, popped *o) {
+  node_hdr *hd = NHDR(pl, 0);
+  o->l1 = hd->l1;
+  o->l2 = hd->l2;
+  o->depth = hd->depth;
+  o->cost = hd->cost;
+  for (int i = 0; i < pl->n; i++) o->perm[i] = NPERM(pl, 0)[i];
+  memcpy(o->ranks, NRANK(pl, 0), pl->n + 1);
+  memmove(pl->mem, pl->mem + pl->stride, (size_t)(pl->size - 1) * pl->stride);
+  pl->size--;
+}
+
+static void *par_worker(void *arg) {
+  par_ctx *P = (par_ctx *)arg;
+  const orc_inst *h = P->h;
+  int n = h->n;
+  int64_t st[ORC_NSTATS] = {0};
+  pool_t pl;
+  pool_init(&pl, n, 1);
+  orc_expansion *x = (orc_expansion *)malloc(sizeof(orc_expansion));
+  popped cur;
+  for (;;) {
+    int got = 0;
+    pthread_mutex_lock(&P->lock);
+    for (;;) {
+      if (pool_pop(P->roots, &cur)) {
+        got = 1;
+        break;
+      }
+      if (P->done) break;
+      if (++P->idle == P->nthreads) { /* no thread holds work: finished */
+        P->done = 1;
+        pthread_cond_broadcast(&P->cond);
+        break;
+      }
+      P->hungry = 1;
+      pthread_cond_wait(&P->cond, &P->lock);
+      P->idle--;
+    }
+    pthread_mutex_unlock(&P->lock);
+    if (!got) break;
+    pool_repush(&pl, &cur);
+    while (pool_pop(&pl, &cur)) {
+      if (P->hungry && pl.size >= 2) { /* give the shallowest node away */
+        popped give;
+        pool_take_bottom(&pl, &give);
+        pthread_mutex_lock(&P->lock);
+        pool_repush(P->roots, &give);
+        P->hungry = P->idle > 1;
+        pthread_cond_signal(&P->cond);
+        pthread_mutex_unlock(&P->lock);
+      }
+      int best = __atomic_load_n(&P->best, __ATOMIC_RELAXED);
+      if (cur.depth == n) {
+        st[2]++;
+        if (cur.cost < best) {
+          pthread_mutex_lock(&P->lock);
+          if (cur.cost < P->best) { /* solutions::insert: strictly better */
+            __atomic_store_n(&P->best, cur.cost, __ATOMIC_RELAXED);
+            P->have = 1;
+            if (P->best_perm) memcpy(P->best_perm, cur.perm, sizeof(int) * n);
+            st[5]++;
+          }
+          pthread_mutex_unlock(&P->lock);
+        }
+      } else if (cur.cost < best) {
+        expand(h, P->kind, cur.perm, cur.l1, cur.l2, cur.depth, x);
+        st[0]++;
+        st[1] += x->nbounded;
+        orc_child **v = x->side == 0 ? x->beg : x->end;
+        int cnt = x->side == 0 ? x->nb : x->ne;
+        for (int i = 0; i < cnt; i++) {
+          unsigned char rk[ORC_MAXN + 1];
+          memcpy(rk, cur.ranks, n + 1);
+          int pos = (cur.depth == n - 2) ? v[i]->depth - 1 : v[i]->depth;
+          rk[pos] = (unsigned char)(cnt - 1 - i);
+          pool_push(&pl, v[i]->perm, v[i]->l1, v[i]->l2, v[i]->depth, v[i]->cost, rk);
+        }
+      } else {
+        st[3]++;
+      }
+    }
+  }
+  free(x);
+  pool_free(&pl);
+  pthread_mutex_lock(&P->lock);
+  for (int i = 0; i < ORC_NSTATS; i++) P->st[i] += st[i];
+  pthread_mutex_unlock(&P->lock);
+  return NULL;
+}
+
+int orc_solve_parallel(const orc_inst *h, int kind, int nthreads, int ub0,
+                       int64_t *stats, int *best_perm) {
+  int n = h->n;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  par_ctx P;
+  memset(&P, 0, sizeof(P));
+  P.h = h;
+  P.kind = kind;
+  P.nthreads = nthreads;
+  P.best = ub0;
+  P.best_perm = best_perm;
+  pthread_mutex_init(&P.lock, NULL);
+  pthread_cond_init(&P.cond, NULL);
+  pool_t roots;
+  pool_init(&roots, n, 1);
+  push_root(&roots, n);
+  P.roots = &roots;
+  /* sequential top: expand level by level (each level in DFS order) until
+   * the frontier holds at least 64 nodes per thread */
+  orc_expansion *x = (orc_expansion *)malloc(sizeof(orc_expansion));
+  int64_t cnt = 1, cap = 1024;
+  popped *fr = (popped *)malloc(sizeof(popped) * cap);
+  pool_pop(&roots, &fr[0]);
+  while (cnt > 0 && cnt < 64 * nthreads) {
+    int64_t ncnt = 0, ncap = 1024;
+    popped *nf = (popped *)malloc(sizeof(popped) * ncap);
+    for (int64_t f = 0; f < cnt; f++) {
+      popped *cur = &fr[f];
+      if (cur->depth == n) {
+        P.st[2]++;
+        if (cur->cost < P.best) {
+          P.best = cur->cost;
+          P.have = 1;
+          P.st[5]++;
+          if (best_perm) memcpy(best_perm, cur->perm, sizeof(int) * n);
+        }
+        continue;
+      }
+      if (cur->cost >= P.best) {
+        P.st[3]++;
+        continue;
+      }
+      expand(h, kind, cur->perm, cur->l1, cur->l2, cur->depth, x);
+      P.st[0]++;
+      P.st[1] += x->nbounded;
+      orc_child **v = x->side == 0 ? x->beg : x->end;
+      int c = x->side == 0 ? x->nb : x->ne;
+      for (int i = c - 1; i >= 0; i--) { /* rank 0 (v[c-1]) first: DFS order */
+        if (ncnt == ncap) {
+          ncap *= 2;
+          nf = (popped *)realloc(nf, sizeof(popped) * ncap);
+        }
+        popped *o = &nf[ncnt++];
+        memcpy(o->perm, v[i]->perm, sizeof(int) * n);
+        memcpy(o->ranks, cur->ranks, n + 1);
+        int pos = (cur->depth == n - 2) ? v[i]->depth - 1 : v[i]->depth;
+        o->ranks[pos] = (unsigned char)(c - 1 - i);
+        o->l1 = v[i]->l1;
+        o->l2 = v[i]->l2;
+        o->depth = v[i]->depth;
+        o->cost = v[i]->cost;
+      }
+    }
+    free(fr);
+    fr = nf;
+    cnt = ncnt;
+  }
+  /* the first subtree in DFS order ends on top of the shared stack */
+  for (int64_t f = cnt - 1; f >= 0; f--) pool_repush(&roots, &fr[f]);
+  free(fr);
+  free(x);
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, par_worker, &P);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  pool_free(&roots);
+  pthread_cond_destroy(&P.cond);
+  pthread_mutex_destroy(&P.lock);
+  if (stats) memcpy(stats, P.st, sizeof(P.st));
+  return P.best;
+}

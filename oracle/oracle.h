@@ -107,6 +107,13 @@ int orc_solve_batched_ex(const orc_inst *h, int kind, int pool_size,
                          int max_batches, int ub0, int64_t budget, int order,
                          int64_t *stats, int *best_perm);
 
+/* all-cores CPU explorer (the CPU baseline's time-to-optimum): the DFS
+ * explorer's rules over independent subtrees of a sequentially expanded
+ * top, nthreads threads, one shared incumbent.  Same optimum as
+ * orc_solve_dfs; node counts depend on thread timing (stats summed). */
+int orc_solve_parallel(const orc_inst *h, int kind, int nthreads, int ub0,
+                       int64_t *stats, int *best_perm);
+
 /* --- search trace (test infrastructure) ---
  * Between orc_trace_begin and orc_trace_end the explorers append one record
  * of 2n + 8 int32 per event to buf (at most cap_records are stored; the

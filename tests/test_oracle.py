@@ -430,3 +430,16 @@ def test_tied_leaves_keep_the_first_found():
             assert o.cmax(bp) == best
             differs += not np.array_equal(tr["perm"][opt[0]], tr["perm"][opt[-1]])
     assert differs > 0
+
+
+def test_parallel_explorer_reaches_the_dfs_optimum():
+    """The all-cores CPU baseline (independent subtrees, shared incumbent,
+    work donation) proves the sequential explorer's optimum."""
+    for tid in (1, 2, 6):
+        o = oracle.Oracle(oracle.taillard(tid)[3])
+        for kind in (oracle.LB2, oracle.LB1):
+            b1, _, _ = o.solve_dfs(kind=kind)
+            for threads in (1, 3, 8):
+                bn, st, bp = o.solve_parallel(kind=kind, threads=threads)
+                assert bn == b1 and o.cmax(bp) == bn
+                assert st["decomposed"] > 0
