@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwsbb_amd.so")
+# WSBB_AMD_LIB names another build of the library (side-by-side timing of
+# two builds in one process tree); unset, the in-tree build is loaded
+LIB_PATH = os.environ.get("WSBB_AMD_LIB") or os.path.join(_HERE, "libwsbb_amd.so")
 
 LB2, LB1, LEARNED = 0, 1, 2
 NSTATS = 12

@@ -60,16 +60,22 @@ struct Rec32 {
   // level (every dword of a batch is used, so it loads as wide scalar loads)
   static constexpr int RS = NW == 1 ? 2 : (NW == 2 ? 3 : 4);
   static constexpr int GW = 4 * RS;  // dwords per four-step group
-  // groups per scalar-load batch: at most 32 SGPRs of records per wait
-  static constexpr int CH = 32 / GW;
+  // groups per scalar-load batch (one wait): up to 48 SGPRs of records for
+  // n <= 32, 64 above (measured best on 20- and 50-job instances)
+  static constexpr int CH = (NW == 1 ? 48 : 64) / GW;
 };
 
 template <int NW, int CH>
 struct Chunk32 {
   int w[CH * Rec32<NW>::GW];
+  // every dword of the batch is made live here, so the batch loads as wide
+  // scalar loads with one wait instead of being split behind the group
+  // branches that follow
   __device__ __forceinline__ void load(const int* p) {
 #pragma unroll
     for (int i = 0; i < CH * Rec32<NW>::GW; ++i) w[i] = ld_const(p, i);
+#pragma unroll
+    for (int i = 0; i < CH * Rec32<NW>::GW; ++i) asm volatile("" ::"s"(w[i]));
   }
 };
 
@@ -97,7 +103,7 @@ __device__ __forceinline__ void group32(const int* g, const uint32_t (&um)[NW], 
 }
 
 // One pair's scan over its gpp groups, CH groups per scalar-load batch
-// (a batch may read past the pair's end: the table holds 16 zero steps
+// (a batch may read past the pair's end: the table holds 32 zero steps
 // after the last pair).
 template <int NW>
 __device__ __forceinline__ void scan_pair32(const DevTables& T, int s, const uint32_t (&um)[NW],
@@ -344,43 +350,20 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     long long fallbacks = 0;
     int npairs = 0;  // pairs scanned by this wave
     if constexpr (MODE == 0) {
-      // every pair of [s_lo, s_hi): the pairs' step groups are contiguous in
-      // the table, so they stream as one sequence, CH groups per scalar-load
-      // batch, with a pair's value taken where its groups end.  Pairs are in
-      // lexicographic order (remplirMachine 500-508), so the next pair's
-      // machines follow without a load.
+      // every pair of [s_lo, s_hi), in lexicographic order (remplirMachine
+      // 500-508): the next pair's machines follow without a load
       npairs = s_hi - s_lo;
-      constexpr int GW = Rec32<NW>::GW, CH = Rec32<NW>::CH;
-      const int gpp = T.n4 >> 2;
-      const int g_lo = s_lo * gpp, g_hi = s_hi * gpp;
-      if (g_lo < g_hi) {
-        const int* gp = T.steps32 + static_cast<size_t>(g_lo) * GW;
+      if (s_lo < s_hi) {
         const int2 ab0 = ld_const2(T.pairs, s_lo);
-        int a = ab0.x, b = ab0.y, kg = 0;
-        int D = st32[a], U = st32[b], one = one_of(a);
-        for (int g = g_lo; g < g_hi; g += CH) {
-          Chunk32<NW, CH> c;
-          c.load(gp);
-          gp += CH * GW;
-#pragma unroll
-          for (int h = 0; h < CH; ++h) {
-            if (g + h < g_hi) {
-              group32<NW>(c.w + h * GW, um, D, U);
-              if (++kg == gpp) {
-                lb = max(lb, pair_value(one, b, U));
-                kg = 0;
-                if (++b == m) {
-                  ++a;
-                  b = a + 1;
-                }
-                // past the range's last pair (a, b) may leave the machines;
-                // the clamp keeps the register index in range
-                const int ac = min(a, MM - 1);
-                D = st32[ac];
-                U = st32[min(b, MM - 1)];
-                one = one_of(ac);
-              }
-            }
+        int a = ab0.x, b = ab0.y;
+        for (int s = s_lo; s < s_hi; ++s) {
+          int D = st32[a], U = st32[b];
+          const int one = one_of(a);
+          scan_pair32<NW>(T, s, um, D, U);
+          lb = max(lb, pair_value(one, b, U));
+          if (++b == m) {
+            ++a;
+            b = a + 1;
           }
         }
       }

@@ -167,10 +167,10 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   // Pack every table into one upload.  steps[s][k] = {job, p_a, p_b,
   // kBias + lag + p_b} in Johnson order, wsel[s][k] the masks that pick the
   // job's 32-bit word of a node's job mask; padding steps are all zero
-  // (no-ops), and 16 more follow the last pair for the scans' read-ahead.
+  // (no-ops), and 32 more follow the last pair for the scans' read-ahead.
   const Instance& I = ctx->inst;
   const int P = I.P, n4 = (n + 3) & ~3;
-  const size_t nsteps = static_cast<size_t>(P) * n4 + 16;
+  const size_t nsteps = static_cast<size_t>(P) * n4 + 32;
   std::vector<int4> steps(nsteps, int4{0, 0, 0, 0});
   std::vector<uint2> wsel(nsteps, uint2{0u, 0u});
   for (int s = 0; s < P; ++s)

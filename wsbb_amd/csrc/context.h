@@ -76,7 +76,7 @@ struct wsbb_ctx {
 
   int grid_cap() const {
     if (launch.grid_cap > 0) return launch.grid_cap;
-    return num_cus * 8;
+    return num_cus * 16;  // 16 blocks per CU: a 2^20-node batch, one per four tiles
   }
   int groups_for(int N) const;
   // two nodes per lane pays once a batch gives every SIMD two 128-node tiles;

@@ -35,7 +35,7 @@ struct DevTables {
   const int32_t* p;      // [m][n]
   const int2* pairs;     // [P] (machine a, machine b)
   const int4* steps;     // [P][n4] {job, p_a, p_b, kBias + lag + p_b}; pad = 0,
-                         // plus 16 zero steps after the last pair
+                         // plus 32 zero steps after the last pair
   const uint2* wsel;     // [P][n4] masks selecting the job's 32-bit mask word:
                          // x = all ones iff bit 0 of (job >> 5), y = bit 1
   const int32_t* R0;     // [m]
@@ -44,7 +44,7 @@ struct DevTables {
   const int* steps32;    // [P][n4] int32 offset-form records of rs dwords:
                          // {32 (p_a - p_b), 32 (lag + p_b) + (job & 31)
                          //  [, word mask of bit 0 of job >> 5, of bit 1]};
-                         // pad = 0, plus 16 zero steps after the last pair
+                         // pad = 0, plus 32 zero steps after the last pair
   int rs;                // dwords per steps32 record: 2 (n <= 32), 3 (n <= 64), 4
   const uint2* steps16;  // [P][n4] two-node form {job, (p_a - p_b mod 2^16)
                          // | (2^14 + lag + p_b) << 16}; pad = 0, plus 16
