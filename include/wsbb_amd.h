@@ -115,6 +115,19 @@ int wsbb_bound_batch_ex(wsbb_ctx *ctx, const uint8_t *permT,
 int wsbb_bound_tiles(wsbb_ctx *ctx, const uint8_t *permT, const int32_t *l1,
                      const int32_t *l2, int N, const uint32_t *tile_sel,
                      int margin, int32_t inc, int32_t *out, int64_t *stats);
+/* No counterpart in the reference: wsbb_bound_batch for callers that cannot
+ * allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * values and errors as wsbb_bound_batch, but the device staging of the
+ * nodes and bounds is a fixed 8 MiB buffer that the library holds on each
+ * GPU from the first wsbb_create on it until the process ends: at most
+ * about 8 MiB / (n + 12) nodes per call (WSBB_EINVAL with the limit in the
+ * message beyond that).  Nothing is allocated or freed.  Every call on one
+ * GPU, from any context, uses that GPU's buffer: calls that overlap in time
+ * on one GPU race (serialise them). */
+int wsbb_bound_batch_noalloc(wsbb_ctx *ctx, const uint8_t *permT,
+                             const int32_t *l1, const int32_t *l2, int N,
+                             int kind, int best, int32_t *out);
 /* Device-resident form: all pointers in HBM, asynchronous on the context's
  * stream (call wsbb_synchronize before reading d_out). */
 int wsbb_bound_device(wsbb_ctx *ctx, const uint8_t *d_permT,

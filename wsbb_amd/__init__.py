@@ -61,6 +61,8 @@ def _load():
         "wsbb_bound_batch_ex": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  _i32p, _i32p, _i32p, _u32p, _f32p], ctypes.c_int),
         "wsbb_bound_device": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+        "wsbb_bound_batch_noalloc": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _i32p], ctypes.c_int),
         "wsbb_bound_tiles": ([_vp, _u8p, _i32p, _i32p, ctypes.c_int, _u32p, ctypes.c_int,
                               ctypes.c_int32, _i32p, _i64p], ctypes.c_int),
         "wsbb_malloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -235,6 +237,19 @@ class Engine:
         if feat:
             res["feat"] = ft
         return res
+
+    def bound_noalloc(self, permT, l1, l2, kind=LB2, best=-1):
+        """wsbb_bound_batch_noalloc: the batch bound staged through the
+        library's fixed per-GPU buffer (no device allocation)."""
+        permT = np.ascontiguousarray(permT, np.uint8)
+        l1 = np.ascontiguousarray(l1, np.int32)
+        l2 = np.ascontiguousarray(l2, np.int32)
+        N = l1.shape[0]
+        assert permT.shape == (self.n, N) and l2.shape == (N,)
+        out = np.zeros(N, np.int32)
+        _check(lib.wsbb_bound_batch_noalloc(self.h, _p(permT, _u8p), _p(l1), _p(l2), N, kind,
+                                            best, _p(out)), "bound_noalloc")
+        return out
 
     def bound_tiles(self, permT, l1, l2, tile_sel, margin=-1, inc=0):
         """LB2 over a pair subset per 64-node tile (+ full-LB2 fallback)."""

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,37 @@ int hip_fail(hipError_t e, const char* what) {
 
 void destroy_search(wsbb_ctx* ctx);    // search.hip
 void destroy_selector(wsbb_ctx* ctx);  // selector.hip
+
+// Staging of wsbb_bound_batch_noalloc: a file-scope device array, so that
+// entry point never allocates.  Each GPU the library runs on holds its own
+// copy until the process ends.  Host code must not take the array's name as
+// a device address: its address on each device is read once, with that
+// device current, by hipGetSymbolAddress and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// with `device` current: resolve that device's copy of the array once
+static int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device index beyond the no-allocation staging table");
+    return WSBB_EINVAL;
+  }
+  if (g_scratch_ptr[device]) return WSBB_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+  return WSBB_OK;
+}
+
+static uint8_t* scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
 
 }  // namespace wsbb
 
@@ -155,6 +187,10 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
   };
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(e, "hipSetDevice");
+  if (int rc = scratch_setup(device)) {
+    wsbb_destroy(ctx);
+    return rc;
+  }
   hipDeviceProp_t prop;
   e = hipGetDeviceProperties(&prop, device);
   if (e != hipSuccess) return fail(e, "hipGetDeviceProperties");
@@ -402,6 +438,54 @@ int wsbb_bound_batch(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
                      const int32_t* l2, int N, int kind, int best, int32_t* out) {
   return wsbb_bound_batch_ex(ctx, permT, l1, l2, N, kind, best, out, nullptr,
                              nullptr, nullptr, nullptr);
+}
+
+int wsbb_bound_batch_noalloc(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
+                             const int32_t* l2, int N, int kind, int best, int32_t* out) {
+  WSBB_CHECK_CTX(ctx);
+  if (N < 0 || (N > 0 && (!permT || !l1 || !l2 || !out)) || (kind != 0 && kind != 1)) {
+    set_error("bad batch arguments");
+    return WSBB_EINVAL;
+  }
+  if (N == 0) return WSBB_OK;
+  const int n = ctx->T.n;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_l1 = al(size_t(n) * N), o_l2 = o_l1 + al(size_t(4) * N);
+  const size_t o_out = o_l2 + al(size_t(4) * N), total = o_out + al(size_t(4) * N);
+  if (total > kScratchBytes) {
+    // the largest N whose staging fits (each segment rounded up to 256 B)
+    const long long nmax = static_cast<long long>(kScratchBytes - 4 * 256) / (n + 12);
+    set_error("wsbb_bound_batch_noalloc: " + std::to_string(N) + " nodes of " +
+              std::to_string(n) + " jobs need " + std::to_string(total) +
+              " bytes of staging; the fixed buffer holds " + std::to_string(kScratchBytes) +
+              " (about " + std::to_string(nmax) + " nodes)");
+    return WSBB_EINVAL;
+  }
+  int rc = check_nodes_host(ctx, permT, l1, l2, N);
+  if (rc) return rc;
+  uint8_t* base = scratch_of(ctx->device);
+  if (!base) {
+    set_error("no-allocation staging was not set up on this device");
+    return WSBB_EHIP;
+  }
+  WSBB_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  WSBB_HIP(hipMemcpyAsync(base, permT, size_t(n) * N, hipMemcpyHostToDevice, st), "H2D perm");
+  WSBB_HIP(hipMemcpyAsync(base + o_l1, l1, size_t(4) * N, hipMemcpyHostToDevice, st), "H2D l1");
+  WSBB_HIP(hipMemcpyAsync(base + o_l2, l2, size_t(4) * N, hipMemcpyHostToDevice, st), "H2D l2");
+  BatchSource src{base, reinterpret_cast<const int32_t*>(base + o_l1),
+                  reinterpret_cast<const int32_t*>(base + o_l2), N};
+  BoundArgs A{};
+  A.N = N;
+  A.kind = kind;
+  A.best = best;
+  A.groups = ctx->groups_for(N);
+  A.pk = ctx->int16_for(N);
+  A.out = reinterpret_cast<int32_t*>(base + o_out);
+  WSBB_HIP(launch_bound(ctx->T, src, A, ctx->grid_cap(), st), "bound kernel launch");
+  WSBB_HIP(hipMemcpyAsync(out, A.out, size_t(4) * N, hipMemcpyDeviceToHost, st), "D2H out");
+  WSBB_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  return WSBB_OK;
 }
 
 int wsbb_bound_device(wsbb_ctx* ctx, const uint8_t* d_permT, const int32_t* d_l1,
