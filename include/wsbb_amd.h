@@ -249,6 +249,53 @@ int wsbb_solve_resume(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
                       const int32_t *best_perm_in, int32_t *best,
                       int32_t *best_perm, int64_t *stats);
 
+/* ---- live node pool -------------------------------------------------------------
+ * The device search driven round by round from the host, with nodes moved
+ * in and out of the pool between rounds.  In the reference the GPU peer's
+ * thread inserts nodes received from other peers into its tree between
+ * pools (Node::recvWorkInWorkStealingGPU3, src/work_stealing/node.cpp
+ * 1240-1283; the GPU thread protocol.cpp:937-1033), gives nodes away
+ * (node.cpp:479-576) and adopts better incumbents broadcast by others
+ * (node.cpp:94-152).  One context is driven by one host thread; several
+ * threads each drive their own context and exchange nodes and incumbents
+ * through these calls.  A pool node is (permutation, limite1, limite2,
+ * bound) with at least two free jobs; node batches are position-major
+ * (permT [n][count]) and bottom-first: the last node of a pushed batch ends
+ * on top of the pool and is decomposed first.
+ *
+ * wsbb_pool_open: a new session of the given kind and pool shape (as
+ * wsbb_solve) with an empty pool, no incumbent (INT32_MAX) and zero
+ * counters.  wsbb_solve leaves its own session open on its final state.
+ * wsbb_pool_seed_root pushes the root (tree::add_root, tree.cpp:170-176). */
+int wsbb_pool_open(wsbb_ctx *ctx, int kind, int pool_size, int max_batches);
+int wsbb_pool_seed_root(wsbb_ctx *ctx);
+/* WSBB_EINVAL for a node with fewer than two free jobs or a column that is
+ * not a permutation; WSBB_EOVERFLOW beyond the pool capacity. */
+int wsbb_pool_push(wsbb_ctx *ctx, const uint8_t *permT, const int32_t *l1,
+                   const int32_t *l2, const int32_t *cost, int count);
+/* Takes up to max nodes off the pool (*count of them): from the top, the
+ * nodes the search would decompose next (tree::take, tree.cpp:257-266), or
+ * with from_bottom != 0 the bottom, the shallowest nodes and so the largest
+ * subtrees (what a peer gives away).  Returned bottom-first; pushing them
+ * back restores the pool. */
+int wsbb_pool_pop(wsbb_ctx *ctx, int max, int from_bottom, uint8_t *permT,
+                  int32_t *l1, int32_t *l2, int32_t *cost, int *count);
+int wsbb_pool_size(wsbb_ctx *ctx, int *size);
+/* The incumbent and its permutation ([n], -1s if unknown; may be NULL).
+ * wsbb_incumbent_set replaces it only by a strictly smaller value
+ * (solutions::insert, src/solutions.cpp:45-65); perm may be NULL. */
+int wsbb_incumbent_get(wsbb_ctx *ctx, int32_t *inc, int32_t *perm);
+int wsbb_incumbent_set(wsbb_ctx *ctx, int32_t inc, const int32_t *perm);
+/* Runs up to `rounds` rounds (<= 0: until the pool is empty) of the open
+ * session; budget bounds the session's decomposed parents (<= 0: none).
+ * *size (may be NULL) receives the pool size after the last round, stats
+ * [WSBB_NSTATS] (may be NULL) the session's counters.  An empty pool means
+ * every node pushed so far is resolved against the incumbent.  Returns
+ * WSBB_EBUDGET or WSBB_EOVERFLOW (pool unchanged by the round that did not
+ * fit) like wsbb_solve; a later call continues. */
+int wsbb_pool_run(wsbb_ctx *ctx, int rounds, int64_t budget, int *size,
+                  int64_t *stats);
+
 /* ---- learned machine-pair selector -------------------------------------------
  * No counterpart in the reference (its pair order ordoSomme is fixed,
  * bound_flowshop.cpp:130 is commented out).  A two-layer MLP (5m features:

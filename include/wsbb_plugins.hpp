@@ -3,22 +3,32 @@
 //
 // It keeps the interfaces a caller of the reference codes against:
 //   struct bound_abstract      (headers/bound_abstract.h:9-19)
+//   struct decompose_abstract  (headers/decompose_abstract.h:11-17)
 //   class select_abstract      (headers/select_abstract.h:4-7)
 //   select_worst               (src/select_worst.cpp:7-12)
 //   class problems             (headers/problems.h:13-28, src/problems.cpp)
 //   decompose_beginend         (headers/decompose_beginend.h:15-44,
-//                               src/decompose_beginend.cpp:26-107)
-// and adds bound_hip, the GPU bound behind bound_abstract.  A per-node
-// bornes_calculer is a batch of one; bornes_calculer_batch bounds a whole
-// vector in one launch, which is how decompose_beginend uses it.  It also
-// reads and writes the reference's text formats (node records, bab.solutions
-// blocks, the formater CSV row) and a pool checkpoint file of the device
-// search built from those records.
+//                               src/decompose_beginend.cpp:20-107)
+// so that the reference's composition root wires them unchanged
+// (pbab::pbab, src/pbab.cpp:49-53: decompose->set_pbab, decompose->set_bound,
+// bound->set_pbab, bound->set_instance), and adds bound_hip, the GPU bound
+// behind bound_abstract.  A per-node bornes_calculer is a batch of one;
+// decompose_beginend bounds both sides of a parent in one launch when its
+// bound is a bound_hip.  gpu_pool drives the device-resident search of a
+// bound_hip's context with the live-pool calls.  The reference's text
+// formats (node records, bab.solutions blocks, the formater CSV row), its
+// binary node record of the work-stealing messages, and a pool checkpoint
+// file of the device search are read and written here too.
+//
+// The node record holds WSBB_MAX_NBJOBS jobs: 20 as in the reference
+// (MAX_NBJOBS, headers/tree.h:31), which makes the binary record 184 bytes;
+// define it before including this header for larger instances (up to 128).
 #ifndef WSBB_PLUGINS_HPP
 #define WSBB_PLUGINS_HPP
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <istream>
 #include <ostream>
@@ -29,10 +39,18 @@
 
 #include "wsbb_amd.h"
 
+#ifndef WSBB_MAX_NBJOBS
+#define WSBB_MAX_NBJOBS 20
+#endif
+
+class pbab;     // the caller's composition root (headers/pbab.h)
+class problem;  // the reference's legacy node class (headers/problem.h)
+
 namespace wsbb_plugins {
 
-constexpr int MAX_NBJOBS = 128;  // the reference fixes 20 (tree.h:31)
-constexpr int BEGIN_ORDER = 0;   // decompose_beginend.cpp:17-18
+constexpr int MAX_NBJOBS = WSBB_MAX_NBJOBS;
+static_assert(MAX_NBJOBS >= 2 && MAX_NBJOBS <= 128, "WSBB_MAX_NBJOBS must be in [2, 128]");
+constexpr int BEGIN_ORDER = 0;  // decompose_beginend.cpp:17-18
 constexpr int END_ORDER = 1;
 
 // Node record with the reference's raw_bb_problem fields
@@ -54,11 +72,30 @@ struct instance_abstract {
 };
 
 struct bound_abstract {
+  pbab* pbb = nullptr;
   virtual void bornes_calculer(int permutation[], int limite1, int limite2, int* couts,
                                int best) = 0;
   virtual void bornes_calculer(raw_bb_problem& p) = 0;
+  void set_pbab(pbab* p) { pbb = p; }
+
+  instance_abstract* instance = nullptr;
   virtual void set_instance(instance_abstract* instance) = 0;
   virtual ~bound_abstract() = default;
+};
+
+class problems;
+
+// decompose_abstract (headers/decompose_abstract.h:11-17).  Its operator()
+// takes the reference's legacy `problem` class, which the device path does
+// not use; the reference's own decompose_beginend does not derive from it
+// either (decompose_beginend.h:15) and neither does the one below.
+struct decompose_abstract {
+  pbab* pbb = nullptr;
+  virtual problems& operator()(problem& n) const = 0;
+
+  bound_abstract* bound = nullptr;
+  virtual void set_bound(bound_abstract* b) = 0;
+  virtual ~decompose_abstract() = default;
 };
 
 class select_abstract {
@@ -86,25 +123,28 @@ class bound_hip : public bound_abstract {
   // bound_flowshop::set_instance -> init (bound_flowshop.cpp:911-927,
   // 953-956): reads N, M, seed and the machine-major times, precomputes and
   // uploads the tables.
-  void set_instance(instance_abstract* instance) override {
-    std::stringstream& in = *instance->data;
+  void set_instance(instance_abstract* inst) override {
+    instance = inst;
+    std::stringstream& in = *inst->data;
     in.clear();
     in.seekg(0);
     long seed = 0;
-    in >> n_ >> m_ >> seed;
-    std::vector<int32_t> p(static_cast<size_t>(n_) * m_);
+    int n = 0, m = 0;
+    in >> n >> m >> seed;
+    if (!in || n < 2 || n > MAX_NBJOBS || m < 2)
+      throw std::runtime_error("bad instance stream or more jobs than WSBB_MAX_NBJOBS");
+    std::vector<int32_t> p(static_cast<size_t>(n) * m);
     for (auto& v : p) in >> v;
     if (!in) throw std::runtime_error("bad instance stream");
-    wsbb_destroy(ctx_);
-    ctx_ = nullptr;
-    check(wsbb_create(p.data(), n_, m_, device_, &ctx_));
+    set_times(p.data(), n, m);
   }
   void set_times(const int32_t* p_machine_major, int n, int m) {
-    n_ = n;
-    m_ = m;
+    if (n > MAX_NBJOBS) throw std::runtime_error("more jobs than WSBB_MAX_NBJOBS");
     wsbb_destroy(ctx_);
     ctx_ = nullptr;
     check(wsbb_create(p_machine_major, n, m, device_, &ctx_));
+    n_ = n;
+    m_ = m;
   }
 
   // bornes_calculer(perm, l1, l2, couts, best), bound_flowshop.cpp:929-946
@@ -142,12 +182,14 @@ class bound_hip : public bound_abstract {
     }
   }
   int size() const { return n_; }
+  int kind() const { return kind_; }
   wsbb_ctx* context() { return ctx_; }
 
- private:
   static void check(int rc) {
     if (rc != WSBB_OK) throw std::runtime_error(std::string("wsbb: ") + wsbb_last_error());
   }
+
+ private:
   int device_, kind_;
   int n_ = 0, m_ = 0;
   wsbb_ctx* ctx_ = nullptr;
@@ -182,63 +224,98 @@ class problems : public std::vector<raw_bb_problem*> {
   }
 };
 
-// decompose_beginend::operator() (decompose_beginend.cpp:91-107): children
-// of both sides (both sides bounded in one launch), the side with more
-// pruned children or, on a tie, the larger bound sum wins (choise 39-50,
-// best fixed at 99999), ranked; returns the chosen side, owned by the
-// decomposer until the next call.
+// decompose_beginend (decompose_beginend.h:15-44): children of both sides
+// (bounded in one launch when the bound is a bound_hip, else one
+// bornes_calculer per child, problems_generate 26-37), the side with more
+// children at or above `best` or, on a tie, the larger bound sum below it
+// wins (choise 39-50; best stays 99999 as in the reference), ranked; the
+// chosen side is returned and owned by the decomposer until the next call.
+// The instance size comes from the bound (bound_hip::size) or set_size.
 class decompose_beginend {
  public:
-  decompose_beginend(bound_hip* bound, int size) : bound_(bound), size_(size) {}
-  ~decompose_beginend() { ns_.release(); }
+  pbab* pbb = nullptr;
+  bound_abstract* bound = nullptr;
+  int best;
+  problems ns;
 
+  explicit decompose_beginend(long unsigned best = 99999) : best(static_cast<int>(best)) {}
+  ~decompose_beginend() { ns.release(); }
+  decompose_beginend(const decompose_beginend&) = delete;
+  decompose_beginend& operator=(const decompose_beginend&) = delete;
+
+  void set_pbab(pbab* p) { pbb = p; }
+  void set_bound(bound_abstract* b) { bound = b; }
+  void set_size(int n) { size_ = n; }
+
+  int choise(problems& debut, problems& fin) const {
+    const int c0 = debut.coupe(best), c1 = fin.coupe(best);
+    if (c0 > c1) return BEGIN_ORDER;
+    if (c1 > c0) return END_ORDER;
+    return debut.sum(best) > fin.sum(best) ? BEGIN_ORDER : END_ORDER;
+  }
+
+  // the children of one side, unbounded (problems_generate 26-37)
+  void problems_generate(problems& out, raw_bb_problem& n, int begin_end) const {
+    out.clear();
+    for (int j = n.limite1 + 1; j < n.limite2; ++j) out.push_back(child(n, j, begin_end, false));
+  }
+  // the two leaves of a parent with two free jobs (leaves_generate 70-89)
+  void leaves_generate(raw_bb_problem& n, problems& out) const {
+    out.clear();
+    for (int t = 1; t <= 2; ++t) out.push_back(child(n, n.limite1 + t, BEGIN_ORDER, true));
+  }
+
+  // decompose_beginend::operator() (decompose_beginend.cpp:91-107)
   problems& operator()(raw_bb_problem& n) {
-    ns_.release();
+    if (!bound) throw std::runtime_error("decompose_beginend: set_bound first");
+    const int sz = size();
+    ns.release();
     problems debut, fin;
-    if (n.depth == size_ - 2) {  // leaves_generate 70-89
-      for (int t = 1; t <= 2; ++t) debut.push_back(child(n, n.limite1 + t, BEGIN_ORDER, true));
-    } else {                     // problems_generate 26-37
-      for (int j = n.limite1 + 1; j < n.limite2; ++j) debut.push_back(child(n, j, BEGIN_ORDER, false));
-      for (int j = n.limite1 + 1; j < n.limite2; ++j) fin.push_back(child(n, j, END_ORDER, false));
+    if (n.depth == sz - 2) {
+      leaves_generate(n, debut);
+    } else {
+      problems_generate(debut, n, BEGIN_ORDER);
+      problems_generate(fin, n, END_ORDER);
     }
     std::vector<raw_bb_problem*> all(debut.begin(), debut.end());
     all.insert(all.end(), fin.begin(), fin.end());
-    bound_->bornes_calculer_batch(all);
+    if (auto* hip = dynamic_cast<bound_hip*>(bound)) {
+      hip->bornes_calculer_batch(all);
+    } else {
+      for (raw_bb_problem* c : all) bound->bornes_calculer(*c);
+    }
     const bool begin_side = choise(debut, fin) == BEGIN_ORDER;
     problems& keep = begin_side ? debut : fin;
     problems& drop = begin_side ? fin : debut;
     drop.release();
-    ns_.assign(keep.begin(), keep.end());
+    ns.assign(keep.begin(), keep.end());
     keep.clear();
-    ns_.ranks(n.depth == size_ - 2);
-    return ns_;
+    ns.ranks(n.depth == sz - 2);
+    return ns;
   }
 
-  int choise(const problems& debut, const problems& fin) const {
-    const int c0 = debut.coupe(best_), c1 = fin.coupe(best_);
-    if (c0 > c1) return BEGIN_ORDER;
-    if (c1 > c0) return END_ORDER;
-    return debut.sum(best_) > fin.sum(best_) ? BEGIN_ORDER : END_ORDER;
+  int size() const {
+    if (size_ > 0) return size_;
+    if (auto* hip = dynamic_cast<const bound_hip*>(bound)) return hip->size();
+    throw std::runtime_error("decompose_beginend: instance size unknown (set_size)");
   }
 
  private:
   // create_raw_problem3 (bb_problem_calculator.cpp:25-87)
   raw_bb_problem* child(const raw_bb_problem& f, int j, int side, bool leaf) const {
+    const int sz = size();
     auto* c = new raw_bb_problem(f);
     c->begin_end = side;
     c->limite1 = f.limite1 + (side == BEGIN_ORDER);
     c->limite2 = f.limite2 - (side == END_ORDER);
     if (c->limite2 - c->limite1 == 2) c->limite1++;
-    c->depth = (f.depth == size_ - 2 || leaf) ? size_ : f.depth + 1;
+    c->depth = (f.depth == sz - 2 || leaf) ? sz : f.depth + 1;
     c->job = f.permutation[j];
     const int t = side == BEGIN_ORDER ? f.limite1 + 1 : f.limite2 - 1;
     std::swap(c->permutation[t], c->permutation[j]);
     return c;
   }
-  bound_hip* bound_;
-  int size_;
-  int best_ = 99999;
-  problems ns_;
+  int size_ = 0;
 };
 
 // ---- text formats ------------------------------------------------------------
@@ -319,16 +396,13 @@ inline void save_pool(std::ostream& os, const pool_snapshot& s) {
   os << '\n';
   for (int32_t v : s.best_perm) os << v << ' ';
   os << '\n';
+  // the record fields of write_record, with n permutation entries
   const int c = s.count();
   for (int k = 0; k < c; ++k) {
-    raw_bb_problem r{};
-    r.couts[0] = r.couts_somme = s.cost[k];
-    for (int i = 0; i < s.n; ++i) r.permutation[i] = s.permT[static_cast<size_t>(i) * c + k];
-    r.limite1 = s.l1[k];
-    r.limite2 = s.l2[k];
-    r.depth = (s.l1[k] + 1) + (s.n - s.l2[k]);
-    r.job = r.begin_end = -1;
-    write_record(os, r, s.n);
+    os << s.cost[k] << ' ' << 0 << ' ' << s.cost[k] << ' ';
+    for (int i = 0; i < s.n; ++i) os << int(s.permT[static_cast<size_t>(i) * c + k]) << ' ';
+    os << s.l1[k] << ' ' << s.l2[k] << ' ' << (s.l1[k] + 1) + (s.n - s.l2[k]) << ' ' << -1 << ' '
+       << -1 << ' ' << '\n';
   }
 }
 inline pool_snapshot load_pool(std::istream& is) {
@@ -336,7 +410,7 @@ inline pool_snapshot load_pool(std::istream& is) {
   std::string tag;
   int count = 0;
   is >> tag >> s.n >> s.inc >> count;
-  if (!is || tag != "wsbb-pool" || s.n < 2 || s.n > MAX_NBJOBS || count < 0)
+  if (!is || tag != "wsbb-pool" || s.n < 2 || s.n > 128 || count < 0)
     throw std::runtime_error("not a wsbb-pool file");
   s.stats.resize(WSBB_NSTATS);
   for (auto& v : s.stats) is >> v;
@@ -347,12 +421,120 @@ inline pool_snapshot load_pool(std::istream& is) {
   s.l2.resize(count);
   s.cost.resize(count);
   for (int k = 0; k < count; ++k) {
+    int c0 = 0, c1 = 0, sum = 0, depth = 0, job = 0, side = 0;
+    is >> c0 >> c1 >> sum;
+    for (int i = 0; i < s.n; ++i) {
+      int v = 0;
+      is >> v;
+      s.permT[static_cast<size_t>(i) * count + k] = static_cast<uint8_t>(v);
+    }
+    is >> s.l1[k] >> s.l2[k] >> depth >> job >> side;
+    s.cost[k] = c0;
+  }
+  if (!is) throw std::runtime_error("truncated wsbb-pool file");
+  return s;
+}
+
+// ---- binary node record of the work-stealing messages ------------------------
+// The reference sends nodes to other peers as consecutive records of int32
+// fields in host byte order: permutation[MAX_NBJOBS], limite1, limite2,
+// couts_somme, depth, begin_end, ranks[MAX_NBJOBS + 1] (the serve side,
+// src/work_stealing/node.cpp:524-556), and the receiver rebuilds a node
+// from each with couts[0] = couts_somme, couts[1] = 0 (node.cpp:1240-1283).
+// With MAX_NBJOBS = 20 a record is 184 bytes.
+constexpr size_t RECORD_BYTES = sizeof(int32_t) * (2 * MAX_NBJOBS + 6);
+
+inline void append_record(std::vector<char>& out, const raw_bb_problem& p) {
+  const size_t at = out.size();
+  out.resize(at + RECORD_BYTES);
+  char* w = out.data() + at;
+  auto put = [&w](const int* v, int cnt) {
+    std::memcpy(w, v, sizeof(int) * cnt);
+    w += sizeof(int) * cnt;
+  };
+  put(p.permutation, MAX_NBJOBS);
+  put(&p.limite1, 1);
+  put(&p.limite2, 1);
+  put(&p.couts_somme, 1);
+  put(&p.depth, 1);
+  put(&p.begin_end, 1);
+  put(p.ranks, MAX_NBJOBS + 1);
+}
+inline std::vector<char> serialize_nodes(const std::vector<const raw_bb_problem*>& nodes) {
+  std::vector<char> out;
+  out.reserve(nodes.size() * RECORD_BYTES);
+  for (const raw_bb_problem* p : nodes) append_record(out, *p);
+  return out;
+}
+// the records of a message; a trailing partial record is an error
+inline std::vector<raw_bb_problem> deserialize_nodes(const char* data, size_t bytes) {
+  if (bytes % RECORD_BYTES != 0) throw std::runtime_error("partial node record");
+  std::vector<raw_bb_problem> out(bytes / RECORD_BYTES);
+  for (raw_bb_problem& p : out) {
+    p = raw_bb_problem{};
+    auto get = [&data](int* v, int cnt) {
+      std::memcpy(v, data, sizeof(int) * cnt);
+      data += sizeof(int) * cnt;
+    };
+    get(p.permutation, MAX_NBJOBS);
+    get(&p.limite1, 1);
+    get(&p.limite2, 1);
+    get(&p.couts_somme, 1);
+    get(&p.depth, 1);
+    get(&p.begin_end, 1);
+    get(p.ranks, MAX_NBJOBS + 1);
+    p.couts[0] = p.couts_somme;
+    p.couts[1] = 0;
+    p.job = -1;
+  }
+  return out;
+}
+
+// A device pool as binary records: the rank strings give the reference's
+// priority queue (tree.cpp:37-46) the device's pop order (ranks[0] = how
+// many nodes are above it on the stack, the rest 0), depth follows from the
+// limits, job and begin_end are -1.
+inline std::vector<char> records_from_pool(const pool_snapshot& s) {
+  if (s.n > MAX_NBJOBS) throw std::runtime_error("more jobs than WSBB_MAX_NBJOBS");
+  const int c = s.count();
+  std::vector<char> out;
+  out.reserve(static_cast<size_t>(c) * RECORD_BYTES);
+  for (int k = 0; k < c; ++k) {
     raw_bb_problem r{};
-    if (!read_record(is, r, s.n)) throw std::runtime_error("truncated wsbb-pool file");
-    for (int i = 0; i < s.n; ++i) s.permT[static_cast<size_t>(i) * count + k] = static_cast<uint8_t>(r.permutation[i]);
-    s.l1[k] = r.limite1;
-    s.l2[k] = r.limite2;
-    s.cost[k] = r.couts[0];
+    for (int i = 0; i < s.n; ++i) r.permutation[i] = s.permT[static_cast<size_t>(i) * c + k];
+    r.limite1 = s.l1[k];
+    r.limite2 = s.l2[k];
+    r.couts[0] = r.couts_somme = s.cost[k];
+    r.depth = (s.l1[k] + 1) + (s.n - s.l2[k]);
+    r.job = r.begin_end = -1;
+    r.ranks[0] = c - 1 - k;
+    append_record(out, r);
+  }
+  return out;
+}
+// Records (any order) as pool nodes, bottom-first: the lexicographically
+// smallest rank string, which the reference's queue pops first, on top.
+inline pool_snapshot pool_from_records(const std::vector<raw_bb_problem>& recs, int n) {
+  if (n < 2 || n > MAX_NBJOBS) throw std::runtime_error("bad job count for node records");
+  std::vector<const raw_bb_problem*> order;
+  for (const auto& r : recs) order.push_back(&r);
+  std::stable_sort(order.begin(), order.end(), [](const raw_bb_problem* a, const raw_bb_problem* b) {
+    return std::lexicographical_compare(b->ranks, b->ranks + MAX_NBJOBS + 1, a->ranks,
+                                        a->ranks + MAX_NBJOBS + 1);
+  });
+  pool_snapshot s;
+  s.n = n;
+  const int c = static_cast<int>(order.size());
+  s.permT.resize(static_cast<size_t>(n) * c);
+  s.l1.resize(c);
+  s.l2.resize(c);
+  s.cost.resize(c);
+  for (int k = 0; k < c; ++k) {
+    for (int i = 0; i < n; ++i)
+      s.permT[static_cast<size_t>(i) * c + k] = static_cast<uint8_t>(order[k]->permutation[i]);
+    s.l1[k] = order[k]->limite1;
+    s.l2[k] = order[k]->limite2;
+    s.cost[k] = order[k]->couts_somme;
   }
   return s;
 }
@@ -385,6 +567,65 @@ inline int resume(wsbb_ctx* ctx, const pool_snapshot& s, int kind, int pool_size
                            s.l2.data(), s.cost.data(), s.count(), s.inc, s.stats.data(),
                            s.best_perm.data(), best, best_perm, stats);
 }
+
+// The device search of a bound_hip's context driven from the host with the
+// live-pool calls (wsbb_pool_*): nodes in and out as pool snapshots or node
+// records, the incumbent, rounds.  The host thread that feeds a GPU in the
+// reference (protocol.cpp:937-1033) becomes a loop over these calls.
+class gpu_pool {
+ public:
+  gpu_pool(bound_hip& bound, int pool_size = 4096, int max_batches = 1)
+      : ctx_(bound.context()), n_(bound.size()) {
+    bound_hip::check(wsbb_pool_open(ctx_, bound.kind(), pool_size, max_batches));
+  }
+  void seed_root() { bound_hip::check(wsbb_pool_seed_root(ctx_)); }
+  void push(const pool_snapshot& s) {
+    bound_hip::check(wsbb_pool_push(ctx_, s.permT.data(), s.l1.data(), s.l2.data(),
+                                    s.cost.data(), s.count()));
+  }
+  void push(const std::vector<raw_bb_problem>& recs) { push(pool_from_records(recs, n_)); }
+  pool_snapshot pop(int max, bool from_bottom) {
+    pool_snapshot s;
+    s.n = n_;
+    s.permT.resize(static_cast<size_t>(n_) * std::max(max, 1));
+    s.l1.resize(std::max(max, 1));
+    s.l2.resize(std::max(max, 1));
+    s.cost.resize(std::max(max, 1));
+    int cnt = 0;
+    bound_hip::check(wsbb_pool_pop(ctx_, max, from_bottom ? 1 : 0, s.permT.data(), s.l1.data(),
+                                   s.l2.data(), s.cost.data(), &cnt));
+    s.permT.resize(static_cast<size_t>(n_) * cnt);  // rows of cnt entries
+    s.l1.resize(cnt);
+    s.l2.resize(cnt);
+    s.cost.resize(cnt);
+    return s;
+  }
+  int size() {
+    int sz = 0;
+    bound_hip::check(wsbb_pool_size(ctx_, &sz));
+    return sz;
+  }
+  int incumbent(std::vector<int32_t>* perm = nullptr) {
+    int32_t inc = 0;
+    if (perm) perm->resize(n_);
+    bound_hip::check(wsbb_incumbent_get(ctx_, &inc, perm ? perm->data() : nullptr));
+    return inc;
+  }
+  void set_incumbent(int32_t inc, const int32_t* perm = nullptr) {
+    bound_hip::check(wsbb_incumbent_set(ctx_, inc, perm));
+  }
+  // runs up to `rounds` rounds (<= 0: until empty); returns the pool size
+  int run(int rounds = 0, int64_t budget = 0, int64_t* stats = nullptr) {
+    int sz = 0;
+    const int rc = wsbb_pool_run(ctx_, rounds, budget, &sz, stats);
+    if (rc != WSBB_OK && rc != WSBB_EBUDGET) bound_hip::check(rc);
+    return sz;
+  }
+
+ private:
+  wsbb_ctx* ctx_;
+  int n_;
+};
 
 // the root of the reference's tree (create_raw_problem 9-22, add_root)
 inline raw_bb_problem* make_root(int size) {

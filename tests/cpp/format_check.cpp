@@ -69,6 +69,32 @@ int main() {
   pool_snapshot t = load_pool(pin);
   ok = ok && t.n == s.n && t.inc == s.inc && t.stats == s.stats && t.best_perm == s.best_perm &&
        t.permT == s.permT && t.l1 == s.l1 && t.l2 == s.l2 && t.cost == s.cost;
+
+  // binary node records: the record's size, a round trip, and its bytes in
+  // hex for comparison with the Python codec
+  for (int i = 0; i <= MAX_NBJOBS; ++i) r.ranks[i] = (i * 3) % 7;
+  std::vector<char> bin = serialize_nodes({&r, &r2});
+  ok = ok && RECORD_BYTES == 184 && bin.size() == 2 * RECORD_BYTES;
+  auto recs = deserialize_nodes(bin.data(), bin.size());
+  ok = ok && recs.size() == 2 && recs[0].couts[0] == r.couts_somme && recs[0].couts[1] == 0;
+  for (int i = 0; ok && i < 20; ++i) ok = recs[0].permutation[i] == r.permutation[i];
+  for (int i = 0; ok && i <= MAX_NBJOBS; ++i) ok = recs[0].ranks[i] == r.ranks[i];
+  ok = ok && recs[1].limite1 == r2.limite1 && recs[1].depth == r2.depth &&
+       recs[1].begin_end == r2.begin_end && recs[1].couts_somme == r2.couts_somme;
+  bool threw = false;
+  try {
+    deserialize_nodes(bin.data(), bin.size() - 4);
+  } catch (const std::runtime_error&) {
+    threw = true;
+  }
+  ok = ok && threw;
+  // a pool as records and back: same nodes, same order
+  pool_snapshot back_pool = pool_from_records(
+      deserialize_nodes(records_from_pool(s).data(), s.count() * RECORD_BYTES), s.n);
+  ok = ok && back_pool.permT == s.permT && back_pool.l1 == s.l1 && back_pool.l2 == s.l2 &&
+       back_pool.cost == s.cost;
+  for (size_t i = 0; i < RECORD_BYTES; ++i) std::printf("%02x", static_cast<unsigned char>(bin[i]));
+  std::printf("\n");
   std::cout << (ok ? "ok" : "MISMATCH") << "\n";
   return ok ? 0 : 1;
 }

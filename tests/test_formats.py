@@ -6,6 +6,8 @@ formater.cpp:94-116), and every written form must read back unchanged."""
 import os
 import subprocess
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -24,4 +26,33 @@ def test_text_formats(tmp_path):
     assert lines[2] == rec
     assert lines[3] == f"1290 0 1290 {perm} 3 17 7 3 1 "
     assert lines[4] == "11;1582;4295624;0.5;"
-    assert lines[5] == "ok"
+    # the first binary record, as the Python codec writes it
+    from wsbb_amd import records
+    r = np.zeros(1, dtype=records.record_dtype())
+    r["permutation"][0] = [(j * 7) % 20 for j in range(20)]
+    r["limite1"], r["limite2"], r["couts_somme"], r["depth"], r["begin_end"] = 3, 17, 1278, 7, 1
+    r["ranks"][0] = [(i * 3) % 7 for i in range(21)]
+    assert len(records.encode(r)) == 184
+    assert lines[5] == records.encode(r).hex()
+    assert lines[6] == "ok"
+
+
+def test_binary_records_round_trip_a_pool():
+    """A device pool written as the reference's binary node records and
+    read back gives the same nodes in the same order, for a shuffled
+    message too (the rank strings carry the pop order)."""
+    from wsbb_amd import records
+    rng = np.random.default_rng(3)
+    n, c = 20, 57
+    ck = dict(permT=np.stack([rng.permutation(n) for _ in range(c)], axis=1).astype(np.uint8),
+              l1=rng.integers(-1, 8, c).astype(np.int32), l2=rng.integers(12, 21, c).astype(np.int32),
+              cost=rng.integers(1000, 2000, c).astype(np.int32))
+    recs = records.from_pool(ck)
+    data = records.encode(recs)
+    assert len(data) == 184 * c
+    back = records.decode(data)
+    shuffled = back[rng.permutation(c)]
+    for got in (records.to_pool(back, n), records.to_pool(shuffled, n)):
+        for k in ("permT", "l1", "l2", "cost"):
+            assert np.array_equal(got[k], ck[k]), k
+    assert (back["depth"] == (ck["l1"] + 1) + (n - ck["l2"])).all()

@@ -81,6 +81,15 @@ def _load():
         "wsbb_set_pool_capacity": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_search_options": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "wsbb_search_trace": ([_vp, _TRACE_FN, _vp], ctypes.c_int),
+        "wsbb_pool_open": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "wsbb_pool_seed_root": ([_vp], ctypes.c_int),
+        "wsbb_pool_push": ([_vp, _u8p, _i32p, _i32p, _i32p, ctypes.c_int], ctypes.c_int),
+        "wsbb_pool_pop": ([_vp, ctypes.c_int, ctypes.c_int, _u8p, _i32p, _i32p, _i32p, _i32p],
+                          ctypes.c_int),
+        "wsbb_pool_size": ([_vp, _i32p], ctypes.c_int),
+        "wsbb_incumbent_get": ([_vp, _i32p, _i32p], ctypes.c_int),
+        "wsbb_incumbent_set": ([_vp, ctypes.c_int32, _i32p], ctypes.c_int),
+        "wsbb_pool_run": ([_vp, ctypes.c_int, ctypes.c_int64, _i32p, _i64p], ctypes.c_int),
         "wsbb_set_int16": ([_vp, ctypes.c_int], ctypes.c_int),
         "wsbb_multi_create": ([ctypes.c_int, ctypes.POINTER(_vp), _i32p, _i32p, ctypes.c_int,
                                ctypes.POINTER(_vp)], ctypes.c_int),
@@ -449,6 +458,58 @@ class Engine:
         self._trace_cb = _TRACE_FN(cb)
         _check(lib.wsbb_search_trace(self.h, self._trace_cb, None), "search_trace")
         return rounds
+
+    # ---- live node pool (wsbb_pool_*) ----
+    def pool_open(self, kind=LB2, pool_size=4096, max_batches=1):
+        _check(lib.wsbb_pool_open(self.h, kind, pool_size, max_batches), "pool_open")
+
+    def pool_seed_root(self):
+        _check(lib.wsbb_pool_seed_root(self.h), "pool_seed_root")
+
+    def pool_push(self, nodes):
+        """nodes: dict permT [n, count], l1, l2, cost (bottom-first)."""
+        permT = np.ascontiguousarray(nodes["permT"], np.uint8)
+        l1, l2, cost = (np.ascontiguousarray(nodes[k], np.int32) for k in ("l1", "l2", "cost"))
+        c = l1.shape[0]
+        assert permT.shape == (self.n, c) and l2.shape == cost.shape == (c,)
+        _check(lib.wsbb_pool_push(self.h, _p(permT, _u8p), _p(l1), _p(l2), _p(cost), c),
+               "pool_push")
+
+    def pool_pop(self, max_nodes, from_bottom=False):
+        permT = np.zeros((self.n, max(max_nodes, 1)), np.uint8)
+        l1, l2, cost = (np.zeros(max(max_nodes, 1), np.int32) for _ in range(3))
+        cnt = ctypes.c_int32()
+        _check(lib.wsbb_pool_pop(self.h, max_nodes, int(from_bottom), _p(permT, _u8p), _p(l1),
+                                 _p(l2), _p(cost), ctypes.byref(cnt)), "pool_pop")
+        c = cnt.value
+        # wsbb_pool_pop writes position-major rows of `c` entries
+        flat = permT.reshape(-1)[: self.n * c].reshape(self.n, c).copy()
+        return dict(permT=flat, l1=l1[:c].copy(), l2=l2[:c].copy(), cost=cost[:c].copy())
+
+    def pool_size(self):
+        sz = ctypes.c_int32()
+        _check(lib.wsbb_pool_size(self.h, ctypes.byref(sz)), "pool_size")
+        return sz.value
+
+    def incumbent(self):
+        inc = ctypes.c_int32()
+        perm = np.zeros(self.n, np.int32)
+        _check(lib.wsbb_incumbent_get(self.h, ctypes.byref(inc), _p(perm)), "incumbent_get")
+        return inc.value, perm
+
+    def set_incumbent(self, inc, perm=None):
+        pp = None if perm is None else np.ascontiguousarray(perm, np.int32)
+        _check(lib.wsbb_incumbent_set(self.h, int(inc), _p(pp)), "incumbent_set")
+
+    def pool_run(self, rounds=0, budget=0):
+        """Runs rounds of the open session; returns (code, pool size,
+        counters) where code is 0, or -5 / -4 for a budget / overflow stop."""
+        sz = ctypes.c_int32()
+        st = np.zeros(NSTATS, np.int64)
+        rc = lib.wsbb_pool_run(self.h, rounds, budget, ctypes.byref(sz), _p(st, _i64p))
+        if rc not in (0, -4, -5):
+            _check(rc, "pool_run")
+        return rc, sz.value, dict(zip(STAT_NAMES, st.tolist()))
 
     def solve(self, kind=LB2, pool_size=4096, max_batches=1, ub0=2**31 - 1, budget=0):
         best = ctypes.c_int32()

@@ -19,11 +19,11 @@ struct ResumeState {
 };
 
 // Runs the search to completion (or budget) with the device node pool;
-// rounds_per_sync rounds are queued between host checks of the pool state.
-// With `resume` the pool, incumbent (ub0) and counters are loaded from it
-// instead of starting from the root.
+// the context's rounds_per_sync rounds are queued between host checks of the
+// pool state.  With `resume` the pool, incumbent (ub0) and counters are
+// loaded from it instead of starting from the root.
 int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
           int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
-          int cap_nodes, int rounds_per_sync, const ResumeState* resume = nullptr);
+          const ResumeState* resume = nullptr);
 
 }  // namespace wsbb

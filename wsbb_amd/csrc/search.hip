@@ -940,7 +940,7 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
 // the list does not matter: every listed child gets its own exact value).
 __global__ __launch_bounds__(256) void k_fallback_list(SArgs S, int margin, int* list) {
   Ctrl* C = S.ctrl;
-  const int nc = C->n_children, inc = C->inc, n = S.n;
+  const int nc = C->n_children, inc = C->inc;
   const int lane = threadIdx.x & 63;
   for (int c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += gridDim.x * blockDim.x) {
     const int c = c0 + threadIdx.x;
@@ -965,25 +965,6 @@ __global__ void k_fallback_stats(SArgs S) {
   Ctrl* C = S.ctrl;
   C->stats[kStFallback] += C->n_fb;
   C->stats[kStPairs] += static_cast<long long>(C->n_fb) * S.P;
-}
-
-__global__ void k_init(SArgs S, int ub0) {
-  Ctrl* C = S.ctrl;
-  const int n = S.n;
-  const int i = threadIdx.x;
-  if (i < n) S.s_perm[static_cast<size_t>(i) * S.CAP] = static_cast<uint8_t>(i);
-  if (i < kMaxJobs) C->best_perm[i] = 0;
-  if (i < WSBB_NSTATS) C->stats[i] = 0;
-  if (i == 0) {
-    S.s_l1[0] = -1;
-    S.s_l2[0] = n;
-    S.s_cost[0] = 0;
-    C->status = kRunning;
-    C->stack_size = 1;
-    C->inc = ub0;
-    C->n_parents = C->n_children = 0;
-    C->have_perm = 0;
-  }
 }
 
 // Records of the plain solver for the pair selector: every stride-th child
@@ -1046,6 +1027,11 @@ struct SearchState {
   std::vector<hipEvent_t> ev;
   double bound_ms = 0;
   long long bound_launches = 0;
+  // the open session: round arguments and search kind of later rounds
+  bool open = false;
+  int kind = WSBB_LB2;
+  SArgs sess{};
+  DevBuf tmp;  // staging for moving pool nodes down (wsbb_pool_pop from the bottom)
 };
 
 static int ensure_aux(wsbb_ctx* ctx) {
@@ -1077,6 +1063,7 @@ void destroy_search(wsbb_ctx* ctx) {
   if (!ctx->search) return;
   for (hipEvent_t e : ctx->search->ev) (void)hipEventDestroy(e);
   if (ctx->search->graph) (void)hipGraphExecDestroy(ctx->search->graph);
+  ctx->search->tmp.release();
   ctx->search->aux.release();
   ctx->search->mem.release();
   if (ctx->search->host_ctrl) (void)hipHostFree(ctx->search->host_ctrl);
@@ -1326,54 +1313,10 @@ static int finish_solve(const wsbb_ctx* ctx, const Ctrl& H, int32_t* best, int32
   return WSBB_OK;
 }
 
-// One round at a time, each followed by host copies of its parents,
-// children and bounds handed to the trace callback.
-static int solve_traced(wsbb_ctx* ctx, const SArgs& A, int kind, int32_t* best,
-                        int32_t* best_perm, int64_t* stats) {
-  SearchState* S = ctx->search;
-  const int n = ctx->T.n;
-  std::vector<uint8_t> permT;
-  std::vector<int32_t> l1, l2, cost, desc, bound;
-  hipError_t e;
-  for (long long round = 0;; ++round) {
-    if ((e = launch_round(ctx, A, kind, nullptr, nullptr)) != hipSuccess)
-      return hip_fail(e, "round launch");
-    Ctrl& H = S->host_ctrl[0];
-    e = hipMemcpyAsync(&H, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "trace control copy");
-    const int np = H.n_parents, nc = H.n_children;
-    if (np > 0) {
-      permT.resize(static_cast<size_t>(n) * np);
-      l1.resize(np);
-      l2.resize(np);
-      cost.resize(np);
-      desc.resize(nc);
-      bound.resize(nc);
-      e = hipMemcpy2DAsync(permT.data(), np, A.p_perm, A.PCAP, np, n, hipMemcpyDeviceToHost,
-                           ctx->stream);
-      const std::pair<int32_t*, const int*> rows[] = {{l1.data(), A.p_l1}, {l2.data(), A.p_l2},
-                                                      {cost.data(), A.p_cost}};
-      for (const auto& r : rows)
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(r.first, r.second, 4 * size_t(np), hipMemcpyDeviceToHost, ctx->stream);
-      if (e == hipSuccess && nc > 0)
-        e = hipMemcpyAsync(desc.data(), A.c_desc, 4 * size_t(nc), hipMemcpyDeviceToHost, ctx->stream);
-      if (e == hipSuccess && nc > 0)
-        e = hipMemcpyAsync(bound.data(), A.c_bound, 4 * size_t(nc), hipMemcpyDeviceToHost,
-                           ctx->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-      if (e != hipSuccess) return hip_fail(e, "trace copy");
-      ctx->trace_fn(ctx->trace_user, round, np, permT.data(), l1.data(), l2.data(), cost.data(),
-                    nc, desc.data(), bound.data(), H.inc);
-    }
-    if (H.status != kRunning) return finish_solve(ctx, H, best, best_perm, stats);
-  }
-}
-
-int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
-          int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
-          int cap_nodes, int rounds_per_sync, const ResumeState* resume) {
+// Opens a search session on the context: the pool shape of later rounds,
+// their buffers and kernel arguments.  The pool's contents are left as they
+// are (callers reset the control block).
+static int open_session(wsbb_ctx* ctx, int kind, int pool_size, int max_batches) {
   const int n = ctx->T.n;
   if (pool_size - 1 <= 2 * n || max_batches < 1 || max_batches > 64 ||
       static_cast<long long>(pool_size) * max_batches > (1 << 23)) {
@@ -1390,7 +1333,7 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   }
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  int rc = ensure_search(ctx, pool_size, max_batches, cap_nodes);
+  int rc = ensure_search(ctx, pool_size, max_batches, ctx->pool_capacity);
   if (rc) return rc;
   SearchState* S = ctx->search;
   if (kind == WSBB_LEARNED || selector_log_stride(ctx) > 0) {
@@ -1400,76 +1343,233 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   SArgs A = S->args;
   A.pool_m1 = pool_size - 1;
   A.max_batches = max_batches;
-  A.budget = budget;
+  A.budget = 0;
   A.count_pairs = kind == WSBB_LB2 ? 1 : kind == WSBB_LEARNED ? 2 : 0;
   A.topk = kind == WSBB_LEARNED ? std::min(selector_topk(ctx), ctx->T.P) : 0;
-  if (resume == nullptr) {
-    hipLaunchKernelGGL(k_init, dim3(1), dim3(kMaxJobs), 0, ctx->stream, A, ub0);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "init launch");
-  } else {
-    const int cnt = resume->count;
-    if (cnt < 0 || cnt > S->CAP) {
-      set_error("saved pool larger than the device pool capacity");
-      return WSBB_EOVERFLOW;
-    }
-    // Every pool node the device makes has at least two free jobs (a child
-    // left with one is completed by limites_set, and leaves are evaluated,
-    // never stored), and each column is a permutation of 0..n-1.
-    for (int k = 0; k < cnt; ++k) {
-      const int a = resume->l1[k], b = resume->l2[k];
-      bool ok = a >= -1 && b <= n && b - a >= 3;
-      uint64_t seen[2] = {0, 0};
-      for (int i = 0; ok && i < n; ++i) {
-        const int j = resume->permT[static_cast<size_t>(i) * cnt + k];
-        ok = j < n && !((seen[j >> 6] >> (j & 63)) & 1u);
-        if (ok) seen[j >> 6] |= uint64_t(1) << (j & 63);
-      }
-      if (!ok) {
-        set_error("saved pool node " + std::to_string(k) +
-                  " has fewer than two free jobs or is not a permutation of 0..n-1");
+  S->sess = A;
+  S->kind = kind;
+  S->open = true;
+  return WSBB_OK;
+}
+
+static int need_session(const wsbb_ctx* ctx) {
+  if (!ctx->search || !ctx->search->open) {
+    set_error("no open search: call wsbb_pool_open or wsbb_solve first");
+    return WSBB_EINVAL;
+  }
+  return WSBB_OK;
+}
+
+// the control block, synchronously (the context's stream is drained)
+static int read_ctrl(wsbb_ctx* ctx, Ctrl* h) {
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h, ctx->search->sess.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? WSBB_OK : hip_fail(e, "control block read");
+}
+static int write_ctrl(wsbb_ctx* ctx, const Ctrl& h) {
+  hipError_t e = hipMemcpyAsync(ctx->search->sess.ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? WSBB_OK : hip_fail(e, "control block write");
+}
+
+// An empty pool with incumbent inc, the given counters (or zeros) and
+// incumbent permutation (or none).
+static int reset_pool(wsbb_ctx* ctx, int32_t inc, const int64_t* stats, const int32_t* perm) {
+  const int n = ctx->T.n;
+  Ctrl h{};
+  h.status = kRunning;
+  h.stack_size = 0;
+  h.inc = inc;
+  if (stats)
+    for (int i = 0; i < WSBB_NSTATS; ++i) h.stats[i] = stats[i];
+  if (perm && perm[0] >= 0) {
+    uint64_t seen[2] = {0, 0};
+    for (int i = 0; i < n; ++i) {
+      const int j = perm[i];
+      if (j < 0 || j >= n || ((seen[j >> 6] >> (j & 63)) & 1u)) {
+        set_error("incumbent permutation is not a permutation of 0..n-1");
         return WSBB_EINVAL;
       }
+      seen[j >> 6] |= uint64_t(1) << (j & 63);
+      h.best_perm[i] = static_cast<uint8_t>(j);
     }
-    if (resume->best_perm && resume->best_perm[0] >= 0) {
-      uint64_t seen[2] = {0, 0};
-      for (int i = 0; i < n; ++i) {
-        const int j = resume->best_perm[i];
-        if (j < 0 || j >= n || ((seen[j >> 6] >> (j & 63)) & 1u)) {
-          set_error("incumbent permutation is not a permutation of 0..n-1");
-          return WSBB_EINVAL;
-        }
-        seen[j >> 6] |= uint64_t(1) << (j & 63);
+    h.have_perm = 1;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return write_ctrl(ctx, h);
+}
+
+// Pushes count nodes (position-major permT [n][count], l1, l2, cost) on top
+// of the pool, bottom-first: node count - 1 ends on top and is taken first.
+// Every pool node the device makes has at least two free jobs (a child left
+// with one is completed by limites_set, and leaves are evaluated, never
+// stored), and each column is a permutation of 0..n-1.
+static int push_nodes(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1, const int32_t* l2,
+                      const int32_t* cost, int count) {
+  const int n = ctx->T.n;
+  for (int k = 0; k < count; ++k) {
+    const int a = l1[k], b = l2[k];
+    bool ok = a >= -1 && b <= n && b - a >= 3;
+    uint64_t seen[2] = {0, 0};
+    for (int i = 0; ok && i < n; ++i) {
+      const int j = permT[static_cast<size_t>(i) * count + k];
+      ok = j < n && !((seen[j >> 6] >> (j & 63)) & 1u);
+      if (ok) seen[j >> 6] |= uint64_t(1) << (j & 63);
+    }
+    if (!ok) {
+      set_error("pool node " + std::to_string(k) +
+                " has fewer than two free jobs or is not a permutation of 0..n-1");
+      return WSBB_EINVAL;
+    }
+  }
+  SearchState* S = ctx->search;
+  Ctrl h{};
+  int rc = read_ctrl(ctx, &h);
+  if (rc) return rc;
+  const int top = h.stack_size;
+  if (static_cast<long long>(top) + count > S->CAP) {
+    set_error("pool push beyond the device pool capacity (" + std::to_string(S->CAP) + " nodes)");
+    return WSBB_EOVERFLOW;
+  }
+  if (count == 0) return WSBB_OK;
+  const SArgs& A = S->sess;
+  hipError_t e = hipMemcpy2DAsync(A.s_perm + top, S->CAP, permT, count, count, n,
+                                  hipMemcpyHostToDevice, ctx->stream);
+  const std::pair<int*, const int32_t*> rows[] = {{A.s_l1, l1}, {A.s_l2, l2}, {A.s_cost, cost}};
+  for (const auto& r : rows)
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r.first + top, r.second, 4 * size_t(count), hipMemcpyHostToDevice,
+                         ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "pool push");
+  h.stack_size = top + count;
+  if (h.status == kDone) h.status = kRunning;
+  return write_ctrl(ctx, h);
+}
+
+// Takes up to max nodes off the pool, from the top (the next nodes the
+// search would decompose) or from the bottom (the shallowest: the largest
+// subtrees); they are returned bottom-first, so pushing them back restores
+// the pool.
+static int pop_nodes(wsbb_ctx* ctx, int max, int from_bottom, uint8_t* permT, int32_t* l1,
+                     int32_t* l2, int32_t* cost, int* count) {
+  const int n = ctx->T.n;
+  SearchState* S = ctx->search;
+  Ctrl h{};
+  int rc = read_ctrl(ctx, &h);
+  if (rc) return rc;
+  const int size = h.stack_size;
+  const int k = std::max(0, std::min(max, size));
+  *count = k;
+  if (k == 0) return WSBB_OK;
+  const SArgs& A = S->sess;
+  const int first = from_bottom ? 0 : size - k;
+  hipError_t e = hipMemcpy2DAsync(permT, k, A.s_perm + first, S->CAP, k, n, hipMemcpyDeviceToHost,
+                                  ctx->stream);
+  const std::pair<int32_t*, const int*> rows[] = {{l1, A.s_l1}, {l2, A.s_l2}, {cost, A.s_cost}};
+  for (const auto& r : rows)
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r.first, r.second + first, 4 * size_t(k), hipMemcpyDeviceToHost,
+                         ctx->stream);
+  if (e == hipSuccess && from_bottom && size > k) {
+    // the nodes above move down by k through a staging buffer (the source
+    // and destination ranges may overlap)
+    const size_t rest = size - k;
+    e = S->tmp.reserve((n + 12) * rest);
+    if (e == hipSuccess) {
+      uint8_t* tp = S->tmp.as<uint8_t>(0);
+      e = hipMemcpy2DAsync(tp, rest, A.s_perm + k, S->CAP, rest, n, hipMemcpyDeviceToDevice,
+                           ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy2DAsync(A.s_perm, S->CAP, tp, rest, rest, n, hipMemcpyDeviceToDevice,
+                             ctx->stream);
+      int* ti = reinterpret_cast<int*>(tp + static_cast<size_t>(n) * rest);
+      int* cols[] = {A.s_l1, A.s_l2, A.s_cost};
+      for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+        e = hipMemcpyAsync(ti, cols[c] + k, 4 * rest, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(cols[c], ti, 4 * rest, hipMemcpyDeviceToDevice, ctx->stream);
       }
     }
-    Ctrl h{};
-    h.status = kRunning;
-    h.stack_size = cnt;
-    h.inc = ub0;
-    if (resume->stats)
-      for (int i = 0; i < WSBB_NSTATS; ++i) h.stats[i] = resume->stats[i];
-    if (resume->best_perm && resume->best_perm[0] >= 0) {
-      h.have_perm = 1;
-      for (int i = 0; i < n; ++i) h.best_perm[i] = static_cast<uint8_t>(resume->best_perm[i]);
-    }
-    e = hipMemcpyAsync(A.ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream);
-    for (int i = 0; i < n && e == hipSuccess && cnt > 0; ++i)
-      e = hipMemcpyAsync(A.s_perm + static_cast<size_t>(i) * S->CAP,
-                         resume->permT + static_cast<size_t>(i) * cnt, cnt,
-                         hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && cnt > 0)
-      e = hipMemcpyAsync(A.s_l1, resume->l1, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && cnt > 0)
-      e = hipMemcpyAsync(A.s_l2, resume->l2, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && cnt > 0)
-      e = hipMemcpyAsync(A.s_cost, resume->cost, 4 * size_t(cnt), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "upload of the saved pool");
   }
-  if (ctx->trace_fn) return solve_traced(ctx, A, kind, best, best_perm, stats);
+  if (e != hipSuccess) return hip_fail(e, "pool pop");
+  h.stack_size = size - k;
+  return write_ctrl(ctx, h);
+}
+
+// Runs rounds of the open session until the pool is empty (or the budget
+// stops it, or it would overflow), or after max_rounds rounds (<= 0: no
+// limit).  budget bounds the decomposed parents counted so far (<= 0:
+// none).  *last receives the final control block.
+static int run_rounds(wsbb_ctx* ctx, long long max_rounds, int64_t budget, Ctrl* last) {
+  SearchState* S = ctx->search;
+  const int kind = S->kind;
+  Ctrl h{};
+  int rc = read_ctrl(ctx, &h);
+  if (rc) return rc;
+  if (h.status != kRunning) {  // a stop by budget or overflow, or an emptied pool
+    h.status = kRunning;
+    if ((rc = write_ctrl(ctx, h))) return rc;
+  }
+  SArgs A = S->sess;
+  A.budget = budget;
+  hipError_t e;
+  const long long limit = max_rounds > 0 ? max_rounds : LLONG_MAX;
+  if (ctx->trace_fn) {
+    // one round at a time, each followed by host copies of its parents,
+    // children and bounds handed to the trace callback
+    const int n = ctx->T.n;
+    std::vector<uint8_t> permT;
+    std::vector<int32_t> l1, l2, cost, desc, bound;
+    Ctrl& H = S->host_ctrl[0];
+    for (long long round = 0; round < limit; ++round) {
+      if ((e = launch_round(ctx, A, kind, nullptr, nullptr)) != hipSuccess)
+        return hip_fail(e, "round launch");
+      e = hipMemcpyAsync(&H, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return hip_fail(e, "trace control copy");
+      const int np = H.n_parents, nc = H.n_children;
+      if (np > 0) {
+        permT.resize(static_cast<size_t>(n) * np);
+        l1.resize(np);
+        l2.resize(np);
+        cost.resize(np);
+        desc.resize(nc);
+        bound.resize(nc);
+        e = hipMemcpy2DAsync(permT.data(), np, A.p_perm, A.PCAP, np, n, hipMemcpyDeviceToHost,
+                             ctx->stream);
+        const std::pair<int32_t*, const int*> rows[] = {{l1.data(), A.p_l1}, {l2.data(), A.p_l2},
+                                                        {cost.data(), A.p_cost}};
+        for (const auto& r : rows)
+          if (e == hipSuccess)
+            e = hipMemcpyAsync(r.first, r.second, 4 * size_t(np), hipMemcpyDeviceToHost,
+                               ctx->stream);
+        if (e == hipSuccess && nc > 0)
+          e = hipMemcpyAsync(desc.data(), A.c_desc, 4 * size_t(nc), hipMemcpyDeviceToHost,
+                             ctx->stream);
+        if (e == hipSuccess && nc > 0)
+          e = hipMemcpyAsync(bound.data(), A.c_bound, 4 * size_t(nc), hipMemcpyDeviceToHost,
+                             ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "trace copy");
+        ctx->trace_fn(ctx->trace_user, round, np, permT.data(), l1.data(), l2.data(),
+                      cost.data(), nc, desc.data(), bound.data(), H.inc);
+      }
+      if (H.status != kRunning) break;
+    }
+    *last = H;
+    return WSBB_OK;
+  }
+  const int rps = ctx->rounds_per_sync;
   const int every = ctx->profile_every;
   const bool timing = every > 0;
-  if (timing && static_cast<int>(S->ev.size()) < 4 * rounds_per_sync) {
-    while (static_cast<int>(S->ev.size()) < 4 * rounds_per_sync) {
+  if (timing && static_cast<int>(S->ev.size()) < 4 * rps) {
+    while (static_cast<int>(S->ev.size()) < 4 * rps) {
       hipEvent_t ev;
       if ((e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
       S->ev.push_back(ev);
@@ -1477,8 +1577,8 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   }
   S->bound_ms = 0;
   S->bound_launches = 0;
-  // hipGraph replay: the kernels of rounds_per_sync rounds are captured once
-  // (their arguments do not change during a solve; all round state lives in
+  // hipGraph replay: the kernels of rps rounds are captured once per call
+  // (their arguments do not change during it; all round state lives in
   // device memory) and replayed until the pool state says stop
   const bool use_graph = ctx->search_graph && !timing;
   if (S->graph) {
@@ -1490,7 +1590,7 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return hip_fail(e, "hipStreamBeginCapture");
     hipError_t le = hipSuccess;
-    for (int r = 0; r < rounds_per_sync && le == hipSuccess; ++r)
+    for (int r = 0; r < rps && le == hipSuccess; ++r)
       le = launch_round(ctx, A, kind, nullptr, nullptr);
     e = hipStreamEndCapture(ctx->stream, &g);
     if (le != hipSuccess) {
@@ -1505,24 +1605,27 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
       return hip_fail(e, "hipGraphInstantiate");
     }
   }
-  // Batches of rounds_per_sync rounds, each followed by a copy of the
-  // control block into its own pinned slot.  The next batch is queued before
-  // the host waits on the current one, so the GPU never idles on a host
-  // round trip; once a batch reports the search finished, the batch behind
-  // it only repeats finished rounds (each returns at once) and is drained.
-  // timed rounds: every `every`-th round of the solve, counted across
-  // batches (the first round of a slot's batch is round base[slot])
+  // Batches of up to rps rounds, each followed by a copy of the control
+  // block into its own pinned slot.  The next batch is queued before the
+  // host waits on the current one, so the GPU never idles on a host round
+  // trip; once a batch reports the search stopped, the batch behind it only
+  // repeats finished rounds (each returns at once) and is drained.  Timed
+  // rounds: every `every`-th round of the call, counted across batches.
   long long queued = 0, base[2] = {0, 0};
+  int nq[2] = {0, 0};
   auto enqueue = [&](int slot) -> hipError_t {
     hipError_t le;
+    const int k = static_cast<int>(std::min<long long>(rps, limit - queued));
+    nq[slot] = k;
+    if (k <= 0) return hipSuccess;
     base[slot] = queued;
-    queued += rounds_per_sync;
-    if (use_graph) {
+    queued += k;
+    if (use_graph && k == rps) {
       if ((le = hipGraphLaunch(S->graph, ctx->stream)) != hipSuccess) return le;
     } else {
-      for (int r = 0; r < rounds_per_sync; ++r) {
+      for (int r = 0; r < k; ++r) {
         hipEvent_t* ev = timing && (base[slot] + r) % every == 0
-                             ? &S->ev[2 * (slot * rounds_per_sync + r)]
+                             ? &S->ev[2 * (slot * rps + r)]
                              : nullptr;
         if ((le = launch_round(ctx, A, kind, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr)) !=
             hipSuccess)
@@ -1535,17 +1638,17 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
     return le;
   };
   // rounds of a batch that did work: the first (rounds counter growth) ones
-  long long prev_rounds = (resume && resume->stats) ? resume->stats[kStRounds] : 0;
+  long long prev_rounds = h.stats[kStRounds];
   auto collect = [&](int slot) -> hipError_t {
     const hipError_t le = hipEventSynchronize(S->copied[slot]);
     if (le != hipSuccess) return le;
     const long long live = S->host_ctrl[slot].stats[kStRounds] - prev_rounds;
     prev_rounds = S->host_ctrl[slot].stats[kStRounds];
     if (!timing) return le;
-    for (int r = 0; r < rounds_per_sync && r < live; ++r) {
+    for (int r = 0; r < nq[slot] && r < live; ++r) {
       if ((base[slot] + r) % every != 0) continue;
       float t = 0;
-      const int i = 2 * (slot * rounds_per_sync + r);
+      const int i = 2 * (slot * rps + r);
       if (hipEventElapsedTime(&t, S->ev[i], S->ev[i + 1]) == hipSuccess) {
         S->bound_ms += t;
         ++S->bound_launches;
@@ -1557,17 +1660,48 @@ int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
   if ((e = enqueue(cur)) != hipSuccess) return hip_fail(e, "round launch");
   for (;;) {
     if ((e = enqueue(cur ^ 1)) != hipSuccess) return hip_fail(e, "round launch");
+    const bool next = nq[cur ^ 1] > 0;
     if ((e = collect(cur)) != hipSuccess) return hip_fail(e, "search sync");
     if (S->host_ctrl[cur].status != kRunning) {
-      // the batch behind repeats finished rounds: wait for it, keep its
-      // (identical) control block and leave its timings out
-      if ((e = hipEventSynchronize(S->copied[cur ^ 1])) != hipSuccess)
+      // the batch behind repeats finished rounds: wait for it, keep this
+      // batch's (identical) control block and leave its timings out
+      if (next && (e = hipEventSynchronize(S->copied[cur ^ 1])) != hipSuccess)
         return hip_fail(e, "search sync");
       break;
     }
+    if (!next) break;  // the round limit
     cur ^= 1;
   }
-  return finish_solve(ctx, S->host_ctrl[cur], best, best_perm, stats);
+  *last = S->host_ctrl[cur];
+  return WSBB_OK;
+}
+
+int solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batches, int32_t ub0,
+          int64_t budget, int32_t* best, int32_t* best_perm, int64_t* stats,
+          const ResumeState* resume) {
+  int rc = open_session(ctx, kind, pool_size, max_batches);
+  if (rc) return rc;
+  const int n = ctx->T.n;
+  if (resume == nullptr) {
+    // the root (create_raw_problem 9-22, tree::add_root tree.cpp:170-176)
+    std::vector<uint8_t> root(n);
+    for (int i = 0; i < n; ++i) root[i] = static_cast<uint8_t>(i);
+    const int32_t a = -1, b = n, c = 0;
+    if ((rc = reset_pool(ctx, ub0, nullptr, nullptr))) return rc;
+    if ((rc = push_nodes(ctx, root.data(), &a, &b, &c, 1))) return rc;
+  } else {
+    if (resume->count < 0 || resume->count > ctx->search->CAP) {
+      set_error("saved pool larger than the device pool capacity");
+      return WSBB_EOVERFLOW;
+    }
+    if ((rc = reset_pool(ctx, ub0, resume->stats, resume->best_perm))) return rc;
+    if ((rc = push_nodes(ctx, resume->permT, resume->l1, resume->l2, resume->cost,
+                         resume->count)))
+      return rc;
+  }
+  Ctrl last{};
+  if ((rc = run_rounds(ctx, 0, budget, &last))) return rc;
+  return finish_solve(ctx, last, best, best_perm, stats);
 }
 
 }  // namespace wsbb
@@ -1591,8 +1725,7 @@ extern "C" int wsbb_solve(wsbb_ctx* ctx, int kind, int pool_size, int max_batche
     wsbb::set_error("null context");
     return WSBB_EINVAL;
   }
-  return wsbb::solve(ctx, kind, pool_size, max_batches, ub0, budget, best, best_perm,
-                     stats, ctx->pool_capacity, ctx->rounds_per_sync);
+  return wsbb::solve(ctx, kind, pool_size, max_batches, ub0, budget, best, best_perm, stats);
 }
 
 extern "C" int wsbb_search_state(wsbb_ctx* ctx, int32_t* inc, int64_t* stats,
@@ -1651,8 +1784,7 @@ extern "C" int wsbb_solve_resume(wsbb_ctx* ctx, int kind, int pool_size, int max
     return WSBB_EINVAL;
   }
   const wsbb::ResumeState R{permT, l1, l2, cost, count, stats_in, best_perm_in};
-  return wsbb::solve(ctx, kind, pool_size, max_batches, inc, budget, best, best_perm, stats,
-                     ctx->pool_capacity, ctx->rounds_per_sync, &R);
+  return wsbb::solve(ctx, kind, pool_size, max_batches, inc, budget, best, best_perm, stats, &R);
 }
 
 extern "C" int wsbb_search_trace(wsbb_ctx* ctx, wsbb_trace_fn fn, void* user) {
@@ -1663,6 +1795,116 @@ extern "C" int wsbb_search_trace(wsbb_ctx* ctx, wsbb_trace_fn fn, void* user) {
   ctx->trace_fn = fn;
   ctx->trace_user = fn ? user : nullptr;
   return WSBB_OK;
+}
+
+extern "C" int wsbb_pool_open(wsbb_ctx* ctx, int kind, int pool_size, int max_batches) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  int rc = wsbb::open_session(ctx, kind, pool_size, max_batches);
+  if (rc) return rc;
+  return wsbb::reset_pool(ctx, INT32_MAX, nullptr, nullptr);
+}
+
+extern "C" int wsbb_pool_seed_root(wsbb_ctx* ctx) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  const int n = ctx->T.n;
+  std::vector<uint8_t> root(n);
+  for (int i = 0; i < n; ++i) root[i] = static_cast<uint8_t>(i);
+  const int32_t a = -1, b = n, c = 0;
+  return wsbb::push_nodes(ctx, root.data(), &a, &b, &c, 1);
+}
+
+extern "C" int wsbb_pool_push(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1,
+                              const int32_t* l2, const int32_t* cost, int count) {
+  if (!ctx || count < 0 || (count > 0 && (!permT || !l1 || !l2 || !cost))) {
+    wsbb::set_error("null context or bad node buffers");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  return wsbb::push_nodes(ctx, permT, l1, l2, cost, count);
+}
+
+extern "C" int wsbb_pool_pop(wsbb_ctx* ctx, int max, int from_bottom, uint8_t* permT,
+                             int32_t* l1, int32_t* l2, int32_t* cost, int* count) {
+  if (!ctx || max < 0 || !count || (max > 0 && (!permT || !l1 || !l2 || !cost))) {
+    wsbb::set_error("null context, negative max or bad node buffers");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  return wsbb::pop_nodes(ctx, max, from_bottom, permT, l1, l2, cost, count);
+}
+
+extern "C" int wsbb_pool_size(wsbb_ctx* ctx, int* size) {
+  if (!ctx || !size) {
+    wsbb::set_error("null context or size");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  wsbb::Ctrl h{};
+  if (int rc = wsbb::read_ctrl(ctx, &h)) return rc;
+  *size = h.stack_size;
+  return WSBB_OK;
+}
+
+extern "C" int wsbb_incumbent_get(wsbb_ctx* ctx, int32_t* inc, int32_t* perm) {
+  if (!ctx || !inc) {
+    wsbb::set_error("null context or incumbent");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  wsbb::Ctrl h{};
+  if (int rc = wsbb::read_ctrl(ctx, &h)) return rc;
+  *inc = h.inc;
+  if (perm)
+    for (int i = 0; i < ctx->T.n; ++i) perm[i] = h.have_perm ? h.best_perm[i] : -1;
+  return WSBB_OK;
+}
+
+extern "C" int wsbb_incumbent_set(wsbb_ctx* ctx, int32_t inc, const int32_t* perm) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  const int n = ctx->T.n;
+  wsbb::Ctrl h{};
+  if (int rc = wsbb::read_ctrl(ctx, &h)) return rc;
+  if (inc >= h.inc) return WSBB_OK;  // only a strictly better value replaces it
+  h.inc = inc;
+  h.have_perm = 0;
+  if (perm) {
+    uint64_t seen[2] = {0, 0};
+    for (int i = 0; i < n; ++i) {
+      const int j = perm[i];
+      if (j < 0 || j >= n || ((seen[j >> 6] >> (j & 63)) & 1u)) {
+        wsbb::set_error("incumbent permutation is not a permutation of 0..n-1");
+        return WSBB_EINVAL;
+      }
+      seen[j >> 6] |= uint64_t(1) << (j & 63);
+      h.best_perm[i] = static_cast<uint8_t>(j);
+    }
+    h.have_perm = 1;
+  }
+  return wsbb::write_ctrl(ctx, h);
+}
+
+extern "C" int wsbb_pool_run(wsbb_ctx* ctx, int rounds, int64_t budget, int* size,
+                             int64_t* stats) {
+  if (!ctx) {
+    wsbb::set_error("null context");
+    return WSBB_EINVAL;
+  }
+  if (int rc = wsbb::need_session(ctx)) return rc;
+  wsbb::Ctrl last{};
+  if (int rc = wsbb::run_rounds(ctx, rounds, budget, &last)) return rc;
+  if (size) *size = last.stack_size;
+  return wsbb::finish_solve(ctx, last, nullptr, nullptr, stats);
 }
 
 extern "C" int wsbb_search_options(wsbb_ctx* ctx, int use_graph, int rounds_per_sync) {
