@@ -16,8 +16,10 @@ with LB2 and LB1 on the GPU and by the CPU explorer), "config2" (ta011, 20x10,
 time-to-optimum with 4096-child pools: ten per round, and one per round as
 the reference launches them), "learned_20x20" and "learned_online" (20x20
 plain against learned pair selection), "mixed_batch" (64 instances of uneven
-size in one launch).  CPU baselines are the oracle (the C restatement of the
-reference CPU path) on this host, on one thread and on all host cores.
+size in one launch), "solve_50x20" (50x20 instances to proven optimum, plain
+and learned, against the CPU explorer).  CPU baselines are the oracle (the C
+restatement of the reference CPU path) on this host, on one thread and on all
+host cores.
 """
 import argparse
 import json
@@ -306,6 +308,10 @@ def bench_learned(wsbb_amd):
 
 
 def online_run(wsbb_amd, eng, pool, nb, warm, cap, topk):
+    return online_run_from(wsbb_amd, eng, pool, nb, warm, cap, topk, dive_first=False)
+
+
+def online_run_from(wsbb_amd, eng, pool, nb, warm, cap, topk, dive_first):
     """One online-learning solve on a context: a plain solve that logs
     records stops after `warm` bounded children, the search is checkpointed,
     the selector is trained from scratch on the records and the search
@@ -315,7 +321,13 @@ def online_run(wsbb_amd, eng, pool, nb, warm, cap, topk):
     eng.selector_config(topk=topk, margin=-1, parents=True)
     eng.selector_log(stride=max(1, warm * eng.n // cap), capacity=cap)
     t = time.perf_counter()
-    first = eng.solve(pool_size=pool, max_batches=nb, budget=warm)
+    if dive_first:
+        # a narrow dive for the first incumbent, then the warm-up from it
+        # (budgets count decomposed parents: about 40 children each here)
+        dive = eng.solve(pool_size=256, max_batches=1, budget=2000)
+        first = eng.solve(pool_size=pool, max_batches=nb, ub0=dive["best"], budget=warm // 40)
+    else:
+        first = eng.solve(pool_size=pool, max_batches=nb, budget=warm)
     bounded = first["stats"]["bounded"]
     train_s = 0.0
     if first["proven"]:
@@ -326,7 +338,8 @@ def online_run(wsbb_amd, eng, pool, nb, warm, cap, topk):
         eng.selector_train(epochs=2, batch=2048, lr=2e-3, seed=2)
         train_s = time.perf_counter() - t2
         eng.selector_log(stride=0, capacity=0)
-        res = eng.resume(ck, kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb)
+        res = eng.resume(ck, kind=wsbb_amd.LEARNED, pool_size=pool, max_batches=nb,
+                         budget=20_000_000 if dive_first else 0)
         bounded = res["stats"]["bounded"]
     return res, time.perf_counter() - t, train_s, bounded
 
@@ -371,6 +384,64 @@ def bench_learned_online(wsbb_amd, ids=tuple(range(21, 31)), runs=3, warm=2_000_
         "instances_faster": int(sum(x > 1.0 for x in sp)),
         "instances": out,
     }
+
+
+def bench_50x20(wsbb_amd, host, cpu=True, seeds=(1, 11, 25, 27, 38, 49, 51, 65, 67, 69, 78)):
+    """Machine-correlated 50x20 instances (p[k][j] = b_k + U[0,20], b_k ~
+    U[1,80]; the seeds among 1-80 whose search does not end at the root)
+    solved to proven optimum: a narrow dive (256-child pools) for the first
+    incumbent, then plain LB2 with 4096-child pools, four per round, and
+    with 65536-child pools, four per round; learned selection online from
+    the same start at the large shape (warm-up of 300000 children, top-24,
+    no fallback, 2e7-parent budget); the CPU explorer on the host's
+    threads.  Job-correlated and Taillard 50x20 instances do not close within
+    the budget."""
+    import oracle
+    rows = []
+    for seed in seeds:
+        p = wsbb_amd.generate_machine_correlated(50, 20, seed)
+        row = {"seed": seed}
+        with wsbb_amd.Engine(p) as eng:
+            eng.solve(pool_size=256, max_batches=1, budget=2000)  # warm up
+            for name, pool, nb in (("plain_4096x4", 4096, 4), ("plain_65536x4", 65536, 4)):
+                t = time.perf_counter()
+                dive = eng.solve(pool_size=256, max_batches=1, budget=2000)
+                r = eng.solve(pool_size=pool, max_batches=nb, ub0=dive["best"], budget=20_000_000)
+                row[name] = {"seconds": time.perf_counter() - t, "optimum": r["best"],
+                             "proven": r["proven"], "bounded": r["stats"]["bounded"]}
+            t = time.perf_counter()
+            res, _, train_s, bounded = online_run_from(wsbb_amd, eng, 65536, 4, 300_000,
+                                                       1 << 17, 24, dive_first=True)
+            row["learned_online_65536x4"] = {"seconds": time.perf_counter() - t,
+                                             "optimum": res["best"], "proven": res["proven"],
+                                             "bounded": bounded, "train_seconds": train_s}
+        if cpu:
+            t = time.perf_counter()
+            b, st, _ = oracle.Oracle(p).solve_parallel(threads=host["threads"])
+            row["cpu_all_threads"] = {"seconds": time.perf_counter() - t, "optimum": b,
+                                      "bounded": st["bounded"], "threads": host["threads"]}
+            row["optimum_match"] = all(row[k]["optimum"] == b for k in
+                                       ("plain_4096x4", "plain_65536x4") if row[k]["proven"]) and \
+                (not row["learned_online_65536x4"]["proven"] or
+                 row["learned_online_65536x4"]["optimum"] == b)
+        rows.append(row)
+    summ = {"instances": len(rows),
+            "plain_proven": sum(r["plain_4096x4"]["proven"] for r in rows),
+            "learned_proven": sum(r["learned_online_65536x4"]["proven"] for r in rows),
+            "learned_faster_than_plain_65536x4": sum(
+                r["learned_online_65536x4"]["proven"] and
+                r["learned_online_65536x4"]["seconds"] < r["plain_65536x4"]["seconds"]
+                for r in rows),
+            "learned_faster_than_plain_4096x4": sum(
+                r["learned_online_65536x4"]["proven"] and
+                r["learned_online_65536x4"]["seconds"] < r["plain_4096x4"]["seconds"]
+                for r in rows),
+            "gpu_plain_4096x4_seconds": sum(r["plain_4096x4"]["seconds"] for r in rows)}
+    if cpu:
+        summ["cpu_all_threads_seconds"] = sum(r["cpu_all_threads"]["seconds"] for r in rows)
+        summ["all_optima_match"] = all(r["optimum_match"] for r in rows)
+    return {"workload": "machine-correlated 50x20 instances solved to proven optimum",
+            "summary": summ, "instances": rows}
 
 
 def bench_mixed(wsbb_amd, budget=100_000_000, host=None):
@@ -502,7 +573,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned"),
+    ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned", "50x20"),
                     default="all",
                     help="batched = the 50x20 launches only (--int16: two-node form); solve = "
                          "ta011 time-to-optimum only; learned = the 20x20 learned comparison")
@@ -538,6 +609,9 @@ def main():
                           "learned_online": bench_learned_online(wsbb_amd,
                                                                  runs=args.online_runs)}),
               flush=True)
+        return
+    if args.workload == "50x20":
+        print(json.dumps(bench_50x20(wsbb_amd, host, cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "solve":
         print(json.dumps(bench_config2(wsbb_amd, host, reps=args.steps, cpu=not args.no_cpu)),
@@ -590,6 +664,7 @@ def main():
         line["learned_20x20"] = bench_learned(wsbb_amd)
         line["learned_online"] = bench_learned_online(wsbb_amd, runs=args.online_runs)
         line["mixed_batch"] = bench_mixed(wsbb_amd, host=None if args.no_cpu else host)
+        line["solve_50x20"] = bench_50x20(wsbb_amd, host, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -59,6 +59,9 @@ int wsbb_taillard(int id, int *n, int *m, int32_t *p);
 int wsbb_generate(int n, int m, long seed, int32_t *p);
 /* job-correlated instance: p[k][j] = a_j + U[0,20], a_j ~ U[1,80] */
 int wsbb_generate_correlated(int n, int m, uint64_t seed, int32_t *p);
+/* machine-correlated instance: p[k][j] = b_k + U[0,20], b_k ~ U[1,80]
+ * (splitmix64 from seed: the m values b_k, then the times machine-major) */
+int wsbb_generate_machine_correlated(int n, int m, uint64_t seed, int32_t *p);
 /* synthetic node batch: node k draws from splitmix64 seeded seed0 + k:
  * d1 ~ U{0..n-1}, d2 ~ U{0..n-1-d1}, Fisher-Yates permutation;
  * l1 = d1 - 1, l2 = n - d2. */

@@ -42,6 +42,8 @@ def lib():
         L.orc_taillard_dims.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.POINTER(ctypes.c_long)]
         L.orc_generate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _i32p]
         L.orc_generate_correlated.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _i32p]
+        L.orc_generate_machine_correlated.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                                      _i32p]
         L.orc_random_nodes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _u8p, _i32p, _i32p]
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [_i32p, ctypes.c_int, ctypes.c_int]
@@ -99,6 +101,12 @@ def generate(n, m, seed):
 def generate_correlated(n, m, seed):
     p = np.zeros((m, n), np.int32)
     lib().orc_generate_correlated(n, m, seed, _p(p))
+    return p
+
+
+def generate_machine_correlated(n, m, seed):
+    p = np.zeros((m, n), np.int32)
+    lib().orc_generate_machine_correlated(n, m, seed, _p(p))
     return p
 
 

@@ -103,6 +103,15 @@ void orc_generate_correlated(int n, int m, uint64_t seed, int *p) {
       p[k * n + j] = a[j] + (int)(orc_splitmix64(&st) % 21);
 }
 
+void orc_generate_machine_correlated(int n, int m, uint64_t seed, int *p) {
+  uint64_t st = seed;
+  int b[ORC_MAXM];
+  for (int k = 0; k < m; k++) b[k] = 1 + (int)(orc_splitmix64(&st) % 80);
+  for (int k = 0; k < m; k++)
+    for (int j = 0; j < n; j++)
+      p[k * n + j] = b[k] + (int)(orc_splitmix64(&st) % 21);
+}
+
 void orc_random_nodes(int n, int N, uint64_t seed0, uint8_t *permT, int *l1,
                       int *l2) {
   int perm[ORC_MAXN + 1];

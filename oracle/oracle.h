@@ -35,6 +35,8 @@ int orc_taillard_dims(int id, int *n, int *m, long *seed);
 void orc_generate(int n, int m, long seed, int *p /* [m*n] machine-major */);
 /* job-correlated generator: p[k][j] = a_j + U[0,20], a_j ~ U[1,80] (splitmix64) */
 void orc_generate_correlated(int n, int m, uint64_t seed, int *p);
+/* machine-correlated generator: p[k][j] = b_k + U[0,20], b_k ~ U[1,80] */
+void orc_generate_machine_correlated(int n, int m, uint64_t seed, int *p);
 uint64_t orc_splitmix64(uint64_t *state);
 /* random node batch: node k uses splitmix64 stream seeded with seed0+k;
  * d1~U{0..n-1}, d2~U{0..n-1-d1}, Fisher-Yates permutation.

@@ -54,6 +54,10 @@ def test_other_generators_match_oracle():
     for n, m, seed in [(20, 5, 1), (50, 20, 77), (100, 10, 2**40 + 3)]:
         assert np.array_equal(wsbb_amd.generate_correlated(n, m, seed),
                               oracle.generate_correlated(n, m, seed))
+        mc = wsbb_amd.generate_machine_correlated(n, m, seed)
+        assert np.array_equal(mc, oracle.generate_machine_correlated(n, m, seed))
+        # a machine's times lie within 20 of each other, in [1, 100]
+        assert (mc.max(axis=1) - mc.min(axis=1) <= 20).all() and mc.min() >= 1 and mc.max() <= 100
     for n, N, s0 in [(20, 1000, 0x5EED), (50, 4096, 0x5EED), (100, 777, 9)]:
         a = wsbb_amd.random_nodes(n, N, s0)
         b = oracle.random_nodes(n, N, s0)

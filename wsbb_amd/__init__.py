@@ -52,6 +52,8 @@ def _load():
         "wsbb_taillard": ([ctypes.c_int, _i32p, _i32p, _i32p], ctypes.c_int),
         "wsbb_generate": ([ctypes.c_int, ctypes.c_int, ctypes.c_long, _i32p], ctypes.c_int),
         "wsbb_generate_correlated": ([ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _i32p], ctypes.c_int),
+        "wsbb_generate_machine_correlated": ([ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _i32p],
+                                             ctypes.c_int),
         "wsbb_random_nodes": ([ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _u8p, _i32p, _i32p], ctypes.c_int),
         "wsbb_precompute": ([_i32p, ctypes.c_int, ctypes.c_int] + [_i32p] * 6, ctypes.c_int),
         "wsbb_create": ([_i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -163,6 +165,12 @@ def generate(n, m, seed):
 def generate_correlated(n, m, seed):
     p = np.zeros((m, n), np.int32)
     _check(lib.wsbb_generate_correlated(n, m, seed, _p(p)))
+    return p
+
+
+def generate_machine_correlated(n, m, seed):
+    p = np.zeros((m, n), np.int32)
+    _check(lib.wsbb_generate_machine_correlated(n, m, seed, _p(p)))
     return p
 
 

@@ -127,6 +127,12 @@ int wsbb_generate_correlated(int n, int m, uint64_t seed, int32_t* p) {
   return WSBB_OK;
 }
 
+int wsbb_generate_machine_correlated(int n, int m, uint64_t seed, int32_t* p) {
+  if (n < 1 || m < 1 || !p) return WSBB_EINVAL;
+  machine_correlated_generate(n, m, seed, p);
+  return WSBB_OK;
+}
+
 int wsbb_random_nodes(int n, int N, uint64_t seed0, uint8_t* permT,
                       int32_t* l1, int32_t* l2) {
   if (n < 2 || n > 255 || N < 0 || !permT || !l1 || !l2) return WSBB_EINVAL;

@@ -79,6 +79,15 @@ void correlated_generate(int n, int m, uint64_t seed, int32_t* p) {
       p[k * n + j] = a[j] + static_cast<int32_t>(splitmix64(&st) % 21);
 }
 
+void machine_correlated_generate(int n, int m, uint64_t seed, int32_t* p) {
+  uint64_t st = seed;
+  std::vector<int32_t> b(m);
+  for (int k = 0; k < m; ++k) b[k] = 1 + static_cast<int32_t>(splitmix64(&st) % 80);
+  for (int k = 0; k < m; ++k)
+    for (int j = 0; j < n; ++j)
+      p[k * n + j] = b[k] + static_cast<int32_t>(splitmix64(&st) % 21);
+}
+
 void random_nodes(int n, int N, uint64_t seed0, uint8_t* permT, int32_t* l1,
                   int32_t* l2) {
   std::vector<int> perm(n);

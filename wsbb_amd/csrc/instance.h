@@ -32,6 +32,8 @@ void taillard_generate(int n, int m, long seed, int32_t* p);
 // instances (p[k][j] = a_j + U[0,20], a_j ~ U[1,80]).
 uint64_t splitmix64(uint64_t* state);
 void correlated_generate(int n, int m, uint64_t seed, int32_t* p);
+// machine-correlated instance: p[k][j] = b_k + U[0,20], b_k ~ U[1,80]
+void machine_correlated_generate(int n, int m, uint64_t seed, int32_t* p);
 void random_nodes(int n, int N, uint64_t seed0, uint8_t* permT, int32_t* l1,
                   int32_t* l2);
 
