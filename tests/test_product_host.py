@@ -83,3 +83,63 @@ def test_create_without_gpu_fails_cleanly():
     with pytest.raises(wsbb_amd.WsbbError) as e:
         wsbb_amd.Engine(oracle.taillard(1)[3])
     assert e.value.code == -6
+
+
+# entry points that take no context, or accept a null one by design
+_NO_CTX = {"wsbb_version", "wsbb_last_error", "wsbb_device_count", "wsbb_taillard",
+           "wsbb_generate", "wsbb_generate_correlated", "wsbb_generate_machine_correlated",
+           "wsbb_random_nodes", "wsbb_precompute", "wsbb_create", "wsbb_destroy",
+           "wsbb_multi_create", "wsbb_multi_destroy"}
+
+
+def test_every_context_entry_point_refuses_a_null_context():
+    """Called with a null context (and null or zero arguments otherwise),
+    every entry point that needs one returns WSBB_EINVAL with a message and
+    touches nothing; no GPU is needed to reach that check."""
+    import ctypes
+    names = [n for n in header_functions() if n not in _NO_CTX]
+    assert len(names) >= 40
+    for name in names:
+        fn = getattr(wsbb_amd.lib, name)
+        args = []
+        for t in fn.argtypes:
+            if t in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
+                     ctypes.c_size_t, ctypes.c_long, ctypes.c_float):
+                args.append(t(0))
+            elif t is wsbb_amd._TRACE_FN:
+                args.append(wsbb_amd._TRACE_FN())
+            else:
+                args.append(None)
+        assert fn(*args) == -1, name
+        assert wsbb_amd.lib.wsbb_last_error(), name
+
+
+def test_create_rejects_bad_instances_before_any_device():
+    """Shapes and times outside the documented limits are refused with
+    WSBB_EINVAL and the reason, whether or not a GPU is present."""
+    import ctypes
+    h = ctypes.c_void_p()
+
+    def create(p, n, m):
+        p = np.ascontiguousarray(p, np.int32)
+        return wsbb_amd.lib.wsbb_create(p.ctypes.data_as(wsbb_amd._i32p), n, m, 0, ctypes.byref(h))
+
+    ok = np.ones((5, 20), np.int32)
+    for p, n, m in ((np.ones((5, 1), np.int32), 1, 5), (np.ones((5, 129), np.int32), 129, 5),
+                    (np.ones((1, 20), np.int32), 20, 1), (np.ones((25, 20), np.int32), 20, 25),
+                    (np.where(np.arange(100).reshape(5, 20) == 7, -1, ok), 20, 5),
+                    (np.where(np.arange(100).reshape(5, 20) == 7, 65536, ok), 20, 5),
+                    (np.full((24, 128), 5000, np.int32), 128, 24)):  # path bound > 2^19
+        assert create(p, n, m) == -1
+        assert wsbb_amd.lib.wsbb_last_error()
+        assert not h.value
+    assert wsbb_amd.lib.wsbb_create(None, 20, 5, 0, ctypes.byref(h)) == -1
+    for bad in (0, 121):
+        assert wsbb_amd.lib.wsbb_taillard(bad, None, None, None) == -1
+    z = np.zeros(8, np.int32)
+    zp = z.ctypes.data_as(wsbb_amd._i32p)
+    assert wsbb_amd.lib.wsbb_generate(0, 5, 1, zp) == -1
+    assert wsbb_amd.lib.wsbb_generate_correlated(5, 0, 1, zp) == -1
+    assert wsbb_amd.lib.wsbb_generate_machine_correlated(5, 5, 1, None) == -1
+    assert wsbb_amd.lib.wsbb_random_nodes(1, 4, 0, None, zp, zp) == -1
+    assert wsbb_amd.lib.wsbb_precompute(None, 20, 5, None, None, None, None, None, None) == -1

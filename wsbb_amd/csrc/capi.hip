@@ -168,20 +168,23 @@ int wsbb_create(const int32_t* p, int n, int m, int device, wsbb_ctx** out) {
     return WSBB_EINVAL;
   }
   *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    set_error("no HIP device available");
-    return WSBB_ENODEV;
-  }
-  if (device < 0 || device >= ndev) {
-    set_error("device index out of range");
-    return WSBB_EINVAL;
-  }
+  // the instance is checked (host only) before any device is touched
   auto* ctx = new wsbb_ctx();
   for (int& g : ctx->tuned_groups) g = -1;
   std::string err = ctx->inst.build(p, n, m);
   if (!err.empty()) {
     set_error(err);
+    delete ctx;
+    return WSBB_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    delete ctx;
+    return WSBB_ENODEV;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("device index out of range");
     delete ctx;
     return WSBB_EINVAL;
   }
