@@ -186,7 +186,6 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
         for (int q = 0; q < 4; ++q)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int h = 32 * bi + 8 * q + 4 * half + r;
             const float y = fmaxf(acc1[bi][4 * q + r], 0.f);
             hb[4 * bi + q][r] = static_cast<short>(f2bf(y));
           }
