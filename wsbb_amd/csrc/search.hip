@@ -96,8 +96,8 @@ struct SArgs {
   int* p_koff;
   int* c_desc;
   int* c_bound;
-  int* c_rank;
-  int* c_scost;
+  int* r_order;     // [PCAP][n] per parent, by rank: the chosen-side child index
+  int* r_cost;      // [PCAP][n] per parent, by rank: that child's bound
   int* p_hr;        // [MM][PCAP + 1] heads of the parent's prefix
   int* p_hq;        // [MM][PCAP + 1] tails of the parent's suffix
   uint32_t* p_hm;   // [NW][PCAP + 1] the parent's scheduled jobs
@@ -621,141 +621,17 @@ hipError_t launch_expand(const wsbb_ctx* ctx, const SArgs& A, int want_state, hi
   return hipErrorInvalidValue;
 }
 
-// ---- rank: one wave per parent -------------------------------------------
-// one DPP step of the leaf minimum: cost ascending, then job descending
-template <int CTRL, int RMASK>
-__device__ __forceinline__ void leaf_step(int& lmin, int& lwin, int& lwjob) {
-  const int om = __builtin_amdgcn_update_dpp(INT_MAX, lmin, CTRL, RMASK, 0xf, false);
-  const int oj = __builtin_amdgcn_update_dpp(-1, lwjob, CTRL, RMASK, 0xf, false);
-  const int ow = __builtin_amdgcn_update_dpp(-1, lwin, CTRL, RMASK, 0xf, false);
-  if (om < lmin || (om == lmin && oj > lwjob)) {
-    lmin = om;
-    lwin = ow;
-    lwjob = oj;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_rank(SArgs S) {
-  __shared__ int s_cost[4][kMaxJobs];
-  __shared__ int s_job[4][kMaxJobs];
-  const Ctrl* C = S.ctrl;
-  const int np = C->n_parents;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = S.n;
-  const int inc0 = C->inc;
-  long long leaves = 0;
-  for (int pb = blockIdx.x * 4; pb < np; pb += gridDim.x * 4) {
-    const int p = pb + wave;
-    const bool live = p < np;
-    int off = 0, cnt = 0, u = 0, simple = 0, side = 0, nch = 0;
-    if (live) {
-      off = S.p_off[p];
-      cnt = S.p_off[p + 1] - off;
-      const int l1 = S.p_l1[p], l2 = S.p_l2[p];
-      u = l2 - l1 - 1;
-      simple = ((l1 + 1) + (n - l2)) == n - 2;
-      int c0 = 0, c1 = 0, s0 = 0, s1 = 0;
-      for (int c = lane; c < cnt; c += 64) {
-        const int b = S.c_bound[off + c];
-        const int sd = simple ? 0 : (c >= u);
-        const int big = b >= kSideBest;
-        if (sd == 0) {
-          c0 += big;
-          s0 += big ? 0 : b;
-        } else {
-          c1 += big;
-          s1 += big ? 0 : b;
-        }
-      }
-      c0 = wave_reduce(c0, 0, OpAdd());
-      c1 = wave_reduce(c1, 0, OpAdd());
-      s0 = wave_reduce(s0, 0, OpAdd());
-      s1 = wave_reduce(s1, 0, OpAdd());
-      side = (c0 > c1) ? 0 : (c1 > c0) ? 1 : (s0 > s1) ? 0 : 1;
-      nch = simple ? (side == 0 ? 2 : 0) : u;
-      const int first = simple ? 0 : side * u;
-      for (int i = lane; i < nch; i += 64) {
-        const int c = first + i;
-        // the i-th child of either side (and of a simple parent) places the
-        // job at position l1 + 1 + i, as expand numbered the descriptors:
-        // no descriptor load in front of the job's
-        const int j = l1 + 1 + i;
-        s_cost[wave][i] = S.c_bound[off + c];
-        s_job[wave][i] = S.p_perm[static_cast<size_t>(j) * S.PCAP + p];
-      }
-    }
-    __syncthreads();
-    if (live) {
-      const int first = simple ? 0 : side * u;
-      int lmin = INT_MAX, lwin = -1, lwjob = -1, lmax = INT_MIN, below = 0;
-      for (int c = lane; c < cnt; c += 64) {
-        const int i = c - first;
-        if (i < 0 || i >= nch) {
-          S.c_rank[off + c] = -1;
-          continue;
-        }
-        const int cc = s_cost[wave][i], jj = s_job[wave][i];
-        int rk = 0;
-        for (int k = 0; k < nch; ++k) {
-          const int ck = s_cost[wave][k];
-          rk += (ck < cc) || (ck == cc && s_job[wave][k] < jj);
-        }
-        S.c_rank[off + c] = rk;
-        S.c_scost[off + rk] = cc;
-        below += cc < inc0;
-        if (simple) {
-          // leaves are inserted worst first (cost desc, job desc) and only a
-          // strictly better one replaces the incumbent: among equal minima
-          // the one with the larger job id wins
-          if (cc < lmin || (cc == lmin && jj > lwjob)) {
-            lmin = cc;
-            lwin = off + c;
-            lwjob = jj;
-          }
-          lmax = max(lmax, cc);
-        }
-      }
-      if (simple) {
-        // scan of (cost ascending, job descending) with the child index as
-        // payload, same DPP steps as wave_incl_scan
-        leaf_step<0x111, 0xf>(lmin, lwin, lwjob);
-        leaf_step<0x112, 0xf>(lmin, lwin, lwjob);
-        leaf_step<0x114, 0xf>(lmin, lwin, lwjob);
-        leaf_step<0x118, 0xf>(lmin, lwin, lwjob);
-        leaf_step<0x142, 0xa>(lmin, lwin, lwjob);
-        leaf_step<0x143, 0xc>(lmin, lwin, lwjob);
-        lmin = __builtin_amdgcn_readlane(lmin, 63);
-        lwin = __builtin_amdgcn_readlane(lwin, 63);
-        lwjob = __builtin_amdgcn_readlane(lwjob, 63);
-        lmax = wave_reduce(lmax, INT_MIN, OpMax());
-      }
-      below = wave_reduce(below, 0, OpAdd());
-      if (lane == 0) {
-        S.p_kept0[p] = below;
-        S.p_side[p] = side;
-        S.p_nch[p] = nch;
-        S.p_leafmin[p] = simple ? lmin : INT_MAX;
-        S.p_leafmax[p] = (simple && nch > 0) ? lmax : INT_MAX;
-        S.p_leafwin[p] = simple ? lwin : -1;
-        if (simple) leaves += nch;
-      }
-    }
-    __syncthreads();
-  }
-  if (lane == 0 && leaves) atomicAdd(reinterpret_cast<unsigned long long*>(&S.ctrl->stats[kStLeaves]),
-                                     static_cast<unsigned long long>(leaves));
-}
-
 // ---- settle: incumbent per parent, kept children, survivor offsets --------
-// One block, parents in chunks of 16384 (thread t owns 16 consecutive
+// One 1024-thread block.  Parents in chunks of 16384 (thread t owns 16 consecutive
 // parents, read and written with 16-byte vector accesses; a round of a
 // 65536-child pool fits one chunk).  The incumbent a parent's children are
 // pruned against is the running minimum of the leaves of the parents popped
 // before it (solutions::insert strictly-better rule, solutions.cpp:45-65;
 // prune loop gpu.cu:317-394).  Its kept children are the ranks whose bound is
 // below that incumbent: counted by the rank kernel against the round's
-// starting incumbent, searched again only for parents after an improvement.
-// Prefix sums of the kept counts place the survivors on the stack.
+// starting incumbent, searched again (in the parent's ranked bounds) only for
+// parents after an improvement.  Prefix sums of the kept counts place the
+// survivors on the stack.
 __device__ __forceinline__ void ld16(const int* a, int (&v)[kSettleK]) {
   const int4* q = reinterpret_cast<const int4*>(a);
 #pragma unroll
@@ -773,17 +649,14 @@ __device__ __forceinline__ void st16(int* a, const int (&v)[kSettleK]) {
   for (int i = 0; i < kSettleK / 4; ++i) q[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
 }
 
-__global__ __launch_bounds__(1024) void k_settle(SArgs S) {
+// np > 0 parents; s_w16 holds 40 ints, s_last one
+__device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
   constexpr int K = kSettleK;
-  __shared__ int s_w16[40];
-  __shared__ int s_last;
   Ctrl* C = S.ctrl;
-  const int np = C->n_parents;
-  if (np == 0) return;
   const int n = S.n, tid = threadIdx.x;
   const int inc0 = C->inc;
   const int sbase = C->stack_size;  // read with the other control fields
-  if (tid == 0) s_last = -1;
+  if (tid == 0) *s_last = -1;
   int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
   int improved = 0, last = -1;
   long long pruned = 0;
@@ -828,11 +701,11 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
           if (ib == inc0) {
             kp = k0[k];
           } else {
-            const int off = S.p_off[p];
+            const int* rc = S.r_cost + static_cast<size_t>(p) * n;
             int lo = 0, hi = nc[k];  // first rank whose bound is >= ib
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
-              if (S.c_scost[off + mid] < ib) lo = mid + 1;
+              if (rc[mid] < ib) lo = mid + 1;
               else hi = mid;
             }
             kp = lo;
@@ -859,7 +732,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
     run_min = min(run_min, cmin);
     run_kept += csum;
   }
-  if (last >= 0) atomicMax(&s_last, last);
+  if (last >= 0) atomicMax(s_last, last);
   for (int d = 32; d >= 1; d >>= 1) {
     improved += __shfl_xor(improved, d, 64);
     pruned += __shfl_xor(pruned, d, 64);
@@ -873,7 +746,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
                 static_cast<unsigned long long>(pruned));
   }
   __syncthreads();
-  const int bl = s_last;
+  const int bl = *s_last;
   if (tid == 0) {
     if (bl >= 0) C->inc = run_min;
     C->push_base = sbase;
@@ -896,42 +769,199 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   }
 }
 
-// ---- push survivors -------------------------------------------------------
-__global__ __launch_bounds__(256) void k_push(SArgs S) {
-  const Ctrl* C = S.ctrl;
-  // every control field in one round of loads, then the overflow test
-  const int status = C->status, nc0 = C->n_children;
-  const int base = C->push_base, total = C->push_total;
-  const int nc = status == kOverflow ? 0 : nc0;
+// ---- rank: one wave per parent -------------------------------------------
+// one DPP step of the leaf minimum: cost ascending, then job descending
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void leaf_step(int& lmin, int& lwin, int& lwjob) {
+  const int om = __builtin_amdgcn_update_dpp(INT_MAX, lmin, CTRL, RMASK, 0xf, false);
+  const int oj = __builtin_amdgcn_update_dpp(-1, lwjob, CTRL, RMASK, 0xf, false);
+  const int ow = __builtin_amdgcn_update_dpp(-1, lwin, CTRL, RMASK, 0xf, false);
+  if (om < lmin || (om == lmin && oj > lwjob)) {
+    lmin = om;
+    lwin = ow;
+    lwjob = oj;
+  }
+}
+
+// Four parents per block and pass.  A parent's children and the jobs at its
+// free positions are read in one round of loads (its child count is 2u, or 2
+// at depth n - 2, as expand numbered them).  The chosen side's ranks go to
+// the parent's ranked rows (child index and bound by rank), which settle and
+// push read without the child descriptors.
+constexpr int kRankWaves = 4;
+
+__global__ __launch_bounds__(256) void k_rank(SArgs S) {
+  __shared__ int s_b[kRankWaves][2 * kMaxJobs];  // children's bounds, BEGIN then END
+  __shared__ int s_job[kRankWaves][kMaxJobs];    // job at free position l1 + 1 + i
+  Ctrl* C = S.ctrl;
+  const int np = C->n_parents;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = S.n;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-    // the child's and its parent's fields load in two rounds, whatever the
-    // child's fate
-    const int rk = S.c_rank[c];
-    const int d = S.c_desc[c];
-    const int p = d >> 9;
-    const int kept = S.p_kept[p], koff = S.p_koff[p], pl1 = S.p_l1[p], pl2 = S.p_l2[p];
-    if (rk < 0 || rk >= kept) continue;
-    const int side = (d >> 8) & 1, j = d & 255;
-    const int slot = base + total - 1 - (koff + rk);
-    int l1 = pl1 + (side == 0), l2 = pl2 - (side == 1);
-    if (l2 - l1 == 2) l1 += 1;
-    const int sw0 = side == 0 ? pl1 + 1 : pl2 - 1;
-    for (int i0 = 0; i0 < n; i0 += 16) {
-      uint32_t v[16];  // one register per byte: every load in flight at once
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int i = i0 + u;
-        const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
-        if (i < n) v[u] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
-      }
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        if (i0 + u < n) S.s_perm[static_cast<size_t>(i0 + u) * S.CAP + slot] = static_cast<uint8_t>(v[u]);
+  const int inc0 = C->inc;
+  long long leaves = 0;
+  for (int pb = blockIdx.x * kRankWaves; pb < np; pb += gridDim.x * kRankWaves) {
+    const int p = pb + wave;
+    const bool live = p < np;
+    int off = 0, l1 = 0, u = 0, simple = 0;
+    if (live) {
+      off = S.p_off[p];
+      l1 = S.p_l1[p];
+      const int l2 = S.p_l2[p];
+      u = l2 - l1 - 1;
+      simple = ((l1 + 1) + (n - l2)) == n - 2;
+      const int cnt = simple ? 2 : 2 * u;
+      for (int c = lane; c < cnt; c += 64) s_b[wave][c] = S.c_bound[off + c];
+      for (int i = lane; i < u; i += 64)
+        s_job[wave][i] = S.p_perm[static_cast<size_t>(l1 + 1 + i) * S.PCAP + p];
     }
-    S.s_l1[slot] = l1;
-    S.s_l2[slot] = l2;
-    S.s_cost[slot] = S.c_bound[c];
+    __syncthreads();
+    if (live) {
+      const int cnt = simple ? 2 : 2 * u;
+      int c0 = 0, c1 = 0, s0 = 0, s1 = 0;
+      for (int c = lane; c < cnt; c += 64) {
+        const int b = s_b[wave][c];
+        const int sd = simple ? 0 : (c >= u);
+        const int big = b >= kSideBest;
+        if (sd == 0) {
+          c0 += big;
+          s0 += big ? 0 : b;
+        } else {
+          c1 += big;
+          s1 += big ? 0 : b;
+        }
+      }
+      c0 = wave_reduce(c0, 0, OpAdd());
+      c1 = wave_reduce(c1, 0, OpAdd());
+      s0 = wave_reduce(s0, 0, OpAdd());
+      s1 = wave_reduce(s1, 0, OpAdd());
+      const int side = (c0 > c1) ? 0 : (c1 > c0) ? 1 : (s0 > s1) ? 0 : 1;
+      const int nch = simple ? (side == 0 ? 2 : 0) : u;
+      // the i-th child of either side (and of a simple parent) places the
+      // job at position l1 + 1 + i
+      const int* sb = &s_b[wave][simple ? 0 : side * u];
+      int* ro = S.r_order + static_cast<size_t>(p) * n;
+      int* rc = S.r_cost + static_cast<size_t>(p) * n;
+      int lmin = INT_MAX, lwin = -1, lwjob = -1, lmax = INT_MIN, below = 0;
+      for (int i = lane; i < nch; i += 64) {
+        const int cc = sb[i], jj = s_job[wave][i];
+        int rk = 0;
+        for (int k = 0; k < nch; ++k) {
+          const int ck = sb[k];
+          rk += (ck < cc) || (ck == cc && s_job[wave][k] < jj);
+        }
+        ro[rk] = i;
+        rc[rk] = cc;
+        below += cc < inc0;
+        if (simple) {
+          // leaves are inserted worst first (cost desc, job desc) and only a
+          // strictly better one replaces the incumbent: among equal minima
+          // the one with the larger job id wins
+          if (cc < lmin || (cc == lmin && jj > lwjob)) {
+            lmin = cc;
+            lwin = off + i;
+            lwjob = jj;
+          }
+          lmax = max(lmax, cc);
+        }
+      }
+      if (simple) {
+        // scan of (cost ascending, job descending) with the child index as
+        // payload, same DPP steps as wave_incl_scan
+        leaf_step<0x111, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x112, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x114, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x118, 0xf>(lmin, lwin, lwjob);
+        leaf_step<0x142, 0xa>(lmin, lwin, lwjob);
+        leaf_step<0x143, 0xc>(lmin, lwin, lwjob);
+        lmin = __builtin_amdgcn_readlane(lmin, 63);
+        lwin = __builtin_amdgcn_readlane(lwin, 63);
+        lwjob = __builtin_amdgcn_readlane(lwjob, 63);
+        lmax = wave_reduce(lmax, INT_MIN, OpMax());
+      }
+      below = wave_reduce(below, 0, OpAdd());
+      if (lane == 0) {
+        S.p_kept0[p] = below;
+        S.p_side[p] = side;
+        S.p_nch[p] = nch;
+        S.p_leafmin[p] = simple ? lmin : INT_MAX;
+        S.p_leafmax[p] = (simple && nch > 0) ? lmax : INT_MAX;
+        S.p_leafwin[p] = simple ? lwin : -1;
+        if (simple) leaves += nch;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && leaves) atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStLeaves]),
+                                     static_cast<unsigned long long>(leaves));
+}
+
+__global__ __launch_bounds__(1024) void k_settle(SArgs S) {
+  __shared__ int s_w16[40];
+  __shared__ int s_last;
+  const int np = S.ctrl->n_parents;
+  if (np == 0) return;
+  settle_round(S, np, s_w16, &s_last);
+}
+
+// ---- push survivors: one wave per parent ------------------------------------
+// A parent's kept children are its first `kept` ranks; the rank-k child goes
+// to stack slot base + total - 1 - (koff + k), so the round's first parent's
+// best child ends on top.  A parent's fields, its permutation column and its
+// ranked rows are read in one round of loads (rows are read whole, n entries,
+// whatever the kept count); lane k then writes the rank-k child.
+__global__ __launch_bounds__(256) void k_push(SArgs S) {
+  __shared__ int s_col[4][kMaxJobs];
+  const Ctrl* C = S.ctrl;
+  // after an overflow or at the end of the search select left no parents
+  const int np = C->n_parents, base = C->push_base, total = C->push_total;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = S.n;
+  for (int pb = blockIdx.x * 4; pb < np; pb += gridDim.x * 4) {
+    const int p = pb + wave;
+    const bool live = p < np;
+    int kept = 0, koff = 0, side = 0, pl1 = 0, pl2 = 0;
+    int ord[2] = {0, 0}, cst[2] = {0, 0};  // ranks lane and lane + 64 (n <= 128)
+    if (live) {
+      kept = S.p_kept[p];
+      koff = S.p_koff[p];
+      side = S.p_side[p];
+      pl1 = S.p_l1[p];
+      pl2 = S.p_l2[p];
+      for (int i = lane; i < n; i += 64)
+        s_col[wave][i] = S.p_perm[static_cast<size_t>(i) * S.PCAP + p];
+      const int* ro = S.r_order + static_cast<size_t>(p) * n;
+      const int* rc = S.r_cost + static_cast<size_t>(p) * n;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int k = lane + 64 * t;
+        if (k < n) {
+          ord[t] = ro[k];
+          cst[t] = rc[k];
+        }
+      }
+    }
+    __syncthreads();
+    if (live) {
+      const int sw0 = side == 0 ? pl1 + 1 : pl2 - 1;
+      int l1 = pl1 + (side == 0), l2 = pl2 - (side == 1);
+      if (l2 - l1 == 2) l1 += 1;  // the last free job is placed too (limites_set)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int k = lane + 64 * t;
+        if (k < kept) {
+          const int j = pl1 + 1 + ord[t];
+          const int slot = base + total - 1 - (koff + k);
+          for (int i = 0; i < n; ++i) {
+            const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
+            S.s_perm[static_cast<size_t>(i) * S.CAP + slot] = static_cast<uint8_t>(s_col[wave][src]);
+          }
+          S.s_l1[slot] = l1;
+          S.s_l2[slot] = l2;
+          S.s_cost[slot] = cst[t];
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1103,8 +1133,11 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   // 16-wide vector accesses past the last parent
   size_t o_pint[13];
   for (int i = 0; i < 13; ++i) o_pint[i] = take(4 * size_t(PCAP + 1 + kSettleK));
-  size_t o_cint[4];
-  for (int i = 0; i < 4; ++i) o_cint[i] = take(4 * size_t(CCAP));
+  size_t o_cint[2];
+  for (int i = 0; i < 2; ++i) o_cint[i] = take(4 * size_t(CCAP));
+  // ranked rows: a parent has at most n chosen-side children
+  const size_t o_rorder = take(4 * size_t(PCAP) * n);
+  const size_t o_rcost = take(4 * size_t(PCAP) * n);
   // parent state rows: padded machine count, job-mask words
   const int m = ctx->T.m;
   const int mmp = m <= 8 ? 8 : (m <= 12 ? 12 : (m <= 16 ? 16 : (m <= 20 ? 20 : 24)));
@@ -1143,8 +1176,10 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
                      &A.p_off,     &A.p_side,    &A.p_nch,     &A.p_leafmin, &A.p_leafmax,
                      &A.p_leafwin, &A.p_kept0,   &A.p_kept,    &A.p_koff};
   for (int i = 0; i < 13; ++i) *pints[i] = S->mem.as<int>(o_pint[i]);
-  int** cints[4] = {&A.c_desc, &A.c_bound, &A.c_rank, &A.c_scost};
-  for (int i = 0; i < 4; ++i) *cints[i] = S->mem.as<int>(o_cint[i]);
+  A.c_desc = S->mem.as<int>(o_cint[0]);
+  A.c_bound = S->mem.as<int>(o_cint[1]);
+  A.r_order = S->mem.as<int>(o_rorder);
+  A.r_cost = S->mem.as<int>(o_rcost);
   A.p_hr = S->mem.as<int>(o_hr);
   A.p_hq = S->mem.as<int>(o_hq);
   A.p_hm = S->mem.as<uint32_t>(o_hm);
@@ -1155,8 +1190,8 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
 static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent_t ev0,
                                hipEvent_t ev1) {
   hipStream_t st = ctx->stream;
-  const int cblocks = (A.CCAP + 255) / 256;
-  const int rblocks = (A.PCAP + 3) / 4;
+  const int rblocks = (A.PCAP + kRankWaves - 1) / kRankWaves;
+  const int pblocks = (A.PCAP + 3) / 4;
   hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
   // parent state for the int32 bound kernel (learned subsets, logging,
   // rounds below the two-node form's batch size); the two-node form of plain
@@ -1288,7 +1323,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   }
   hipLaunchKernelGGL(k_rank, dim3(rblocks < 4096 ? rblocks : 4096), dim3(256), 0, st, A);
   hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
-  hipLaunchKernelGGL(k_push, dim3(cblocks), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_push, dim3(pblocks < 4096 ? pblocks : 4096), dim3(256), 0, st, A);
   return hipGetLastError();
 }
 
