@@ -795,8 +795,8 @@ hipError_t launch_pk_src(const DevTables& T, const Src& src, const BoundArgs& A,
   return hipErrorInvalidValue;
 }
 
-// Selector features of every node as BF16 B-operand fragments of
-// v_mfma_f32_32x32x8_bf16.  5m features per node: per machine the head (R0
+// Selector features of every node as BF16 B-operand fragments in the
+// 8-wide MFMA layout (two of them make one 32x32x16 operand).  5m features per node: per machine the head (R0
 // without prefix), the tail (Q0 without suffix), the remaining work of the
 // free jobs, head + work and work + tail, in that order of groups, divided by
 // `scale`.  Lane half*32 + c of the fragment of 32-node block b and feature

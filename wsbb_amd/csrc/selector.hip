@@ -3,7 +3,7 @@
 //
 // Inference (the search's hot path) runs on MFMA.  It computes the transposed
 // product: weights are the A operand and node features the B operand of
-// v_mfma_f32_32x32x8_bf16, so the layer-1 accumulator of a lane holds exactly
+// v_mfma_f32_32x32x16_bf16, so the layer-1 accumulator of a lane holds exactly
 // the hidden units that lane must supply as layer-2 B operand; the two layers
 // chain in registers.  One wavefront handles one 64-node tile, turns the
 // scores into per-node softmax probabilities, sums them over the tile and
@@ -35,8 +35,9 @@ namespace {
 
 constexpr int kHidden = 128;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
   uint32_t u;
@@ -77,13 +78,15 @@ struct MlpArgs {
   int ldp;                  // multiple of 4, >= 32 PB
 };
 
-__device__ __forceinline__ bf16x4 as_bf16x4(uint2 a) {
-  bf16x4 r;
-  r[0] = static_cast<short>(a.x & 0xffff);
-  r[1] = static_cast<short>(a.x >> 16);
-  r[2] = static_cast<short>(a.y & 0xffff);
-  r[3] = static_cast<short>(a.y >> 16);
-  return r;
+// One operand of v_mfma_f32_32x32x16_bf16 from two 8-wide fragments of the
+// same lane (four BF16 values each, the 32x32x8 layout the feature and
+// weight writers produce).  A lane's eight k values of the 16-wide step are
+// its four of step 2t followed by its four of step 2t + 1; A and B operands
+// are both built this way, so the product sums the same 16 k terms, only in
+// another order.
+__device__ __forceinline__ bf16x8 cat_bf16(uint2 lo, uint2 hi) {
+  const u32x4 v = {lo.x, lo.y, hi.x, hi.y};
+  return __builtin_bit_cast(bf16x8, v);
 }
 
 // One persistent block per CU with eight waves (four when the LDS cannot hold
@@ -168,26 +171,34 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc1[bi][v] = A.b1[32 * bi + 8 * (v >> 2) + 4 * half + (v & 3)];
+      // 16 features per step; an odd last step pairs with zeros (in both
+      // operands, so nothing past the fragments is read)
+      const uint2 z2 = make_uint2(0u, 0u);
 #pragma unroll 1
-      for (int t = 0; t < A.fsteps; ++t) {
-        const bf16x4 b = as_bf16x4(s_x[t * 64 + lane]);
+      for (int t = 0; t < A.fsteps; t += 2) {
+        const bool two = t + 1 < A.fsteps;
+        const bf16x8 b = cat_bf16(s_x[t * 64 + lane], two ? s_x[(t + 1) * 64 + lane] : z2);
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
-          acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-              as_bf16x4(w1[(bi * A.fsteps + t) * 64 + lane]), b, acc1[bi], 0, 0, 0);
+        for (int bi = 0; bi < 4; ++bi) {
+          const uint2* wr = w1 + (bi * A.fsteps + t) * 64 + lane;
+          acc1[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cat_bf16(wr[0], two ? wr[64] : z2), b,
+                                                             acc1[bi], 0, 0, 0);
+        }
       }
       __builtin_amdgcn_wave_barrier();  // s_x is reused for s_p below
       // bias, ReLU, BF16: lane holds hidden 32bi + 8q + 4half + r for its node,
-      // which is the layer-2 B operand of k-step 4bi + q
-      bf16x4 hb[16];
+      // which is element r of the layer-2 B fragment of 8-wide step 4bi + q,
+      // so element 4(q & 1) + r of 16-wide step 2bi + q/2 (packed two to a word)
+      uint32_t hw[8][4];
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float y = fmaxf(acc1[bi][4 * q + r], 0.f);
-            hb[4 * bi + q][r] = static_cast<short>(f2bf(y));
+          for (int r = 0; r < 4; r += 2) {
+            const uint32_t lo = f2bf(fmaxf(acc1[bi][4 * q + r], 0.f));
+            const uint32_t hi = f2bf(fmaxf(acc1[bi][4 * q + r + 1], 0.f));
+            hw[2 * bi + (q >> 1)][2 * (q & 1) + (r >> 1)] = lo | (hi << 16);
           }
       // layer 2: Z^T[PB*32 x 32] = W2^T . H^T
       f32x16 acc2[PB];
@@ -198,20 +209,32 @@ __global__ __launch_bounds__(512) void k_mlp_select(MlpArgs A) {
       // fully unrolled (hb is indexed statically); the next step's weight
       // fragments are read while this step multiplies, and the scheduling
       // barrier keeps the compiler from hoisting further reads
-      uint2 wa[PB], wb[PB];
+      uint2 wa[PB][2], wb[PB][2];
 #pragma unroll
-      for (int bp = 0; bp < PB; ++bp) wa[bp] = w2[(bp * 16) * 64 + lane];
+      for (int bp = 0; bp < PB; ++bp) {
+        wa[bp][0] = w2[(bp * 16) * 64 + lane];
+        wa[bp][1] = w2[(bp * 16 + 1) * 64 + lane];
+      }
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        if (t + 1 < 16) {
+      for (int t = 0; t < 8; ++t) {
+        if (t + 1 < 8) {
 #pragma unroll
-          for (int bp = 0; bp < PB; ++bp) wb[bp] = w2[(bp * 16 + t + 1) * 64 + lane];
+          for (int bp = 0; bp < PB; ++bp) {
+            wb[bp][0] = w2[(bp * 16 + 2 * t + 2) * 64 + lane];
+            wb[bp][1] = w2[(bp * 16 + 2 * t + 3) * 64 + lane];
+          }
         }
+        const u32x4 hv = {hw[t][0], hw[t][1], hw[t][2], hw[t][3]};
+        const bf16x8 hb = __builtin_bit_cast(bf16x8, hv);
 #pragma unroll
         for (int bp = 0; bp < PB; ++bp)
-          acc2[bp] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_bf16x4(wa[bp]), hb[t], acc2[bp], 0, 0, 0);
+          acc2[bp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cat_bf16(wa[bp][0], wa[bp][1]), hb,
+                                                             acc2[bp], 0, 0, 0);
 #pragma unroll
-        for (int bp = 0; bp < PB; ++bp) wa[bp] = wb[bp];
+        for (int bp = 0; bp < PB; ++bp) {
+          wa[bp][0] = wb[bp][0];
+          wa[bp][1] = wb[bp][1];
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
       // logits; this lane's pairs are 32bp + 8(v/4) + 4half + v%4
