@@ -12,8 +12,10 @@ from tracecheck import rebuild_children, taken
 pytestmark = pytest.mark.gpu
 
 
-def traced_solve(p, pool, nb, kind=wsbb_amd.LB2, **kw):
+def traced_solve(p, pool, nb, kind=wsbb_amd.LB2, int16=None, **kw):
     with wsbb_amd.Engine(p) as eng:
+        if int16 is not None:
+            eng.set_int16(int16)
         rounds = eng.search_trace()
         res = eng.solve(kind=kind, pool_size=pool, max_batches=nb, **kw)
         eng.search_trace(False)
@@ -34,6 +36,34 @@ def test_ta011_config_every_child_bound_matches_oracle():
     for k in ("decomposed", "bounded", "leaves", "pruned_pop", "pruned_insert",
               "improvements", "max_pool", "rounds"):
         assert res["stats"][k] == st[k], k
+    total = 0
+    for rnd in rounds:
+        permT, l1, l2 = rebuild_children(rnd, n)
+        ref = o.bound_batch(permT, l1, l2, threads=8)
+        bad = np.flatnonzero(ref != rnd["bound"])
+        assert bad.size == 0, (rnd["round"], bad[:5], ref[bad[:5]], rnd["bound"][bad[:5]])
+        total += l1.shape[0]
+    assert total == st["bounded"]
+
+
+@pytest.mark.parametrize("tid,pool,nb,budget", [(11, 4096, 1, 0), (21, 65536, 4, 300_000),
+                                                (51, 4096, 10, 40_000)])
+def test_two_node_search_rounds_match_oracle(tid, pool, nb, budget):
+    """Search rounds forced through the two-node int16 form: every child of
+    every round against the oracle's bornes_calculer, on a 20x10 solve to the
+    optimum, a budgeted 20x20 solve with the 2^18-child rounds that select
+    this form by default, and a budgeted 50x20 solve, where each 128-child
+    tile scans only the union of its children's free jobs (two mask words
+    per node)."""
+    p = oracle.taillard(tid)[3]
+    n = p.shape[1]
+    o = oracle.Oracle(p)
+    best, st, _ = o.solve_batched(pool_size=pool, max_batches=nb, budget=budget)
+    res, rounds = traced_solve(p, pool, nb, int16=2, budget=budget)
+    # no leaf reached before the budget: the oracle says -1, the device INT32_MAX
+    assert res["best"] == (best if best >= 0 else 2**31 - 1)
+    assert res["stats"]["bounded"] == st["bounded"]
+    assert res["stats"]["decomposed"] == st["decomposed"]
     total = 0
     for rnd in rounds:
         permT, l1, l2 = rebuild_children(rnd, n)

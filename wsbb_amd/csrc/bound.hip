@@ -506,11 +506,10 @@ __device__ __forceinline__ uint32_t as_u32(s16x2 x) { return __builtin_bit_cast(
 // The flags: the job's bit of each node's 32-bit mask word (word chosen by
 // the step's select masks, as in the one-node form), packed into the halves.
 template <int NW>
-__device__ __forceinline__ void step4_pk(const uint2 (&rec)[4], const uint32_t (&uA)[NW],
+__device__ __forceinline__ void step1_pk(const uint2 rec, const uint32_t (&uA)[NW],
                                          const uint32_t (&uB)[NW], s16x2& D, s16x2& u) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t j = rec[k].x;  // job: bit offset in its low 5 bits, word above
+  {
+    const uint32_t j = rec.x;  // job: bit offset in its low 5 bits, word above
     uint32_t wA, wB;
     if constexpr (NW == 1) {
       wA = uA[0];
@@ -531,11 +530,34 @@ __device__ __forceinline__ void step4_pk(const uint2 (&rec)[4], const uint32_t (
     const s16x2 f = as_s16((fB << 16) | fA);
     // one word holds both constants: p_a - p_b (low half), 2^14 + lag + p_b
     // (high half); each multiply-add reads its half in both lanes
-    const s16x2 c = as_s16(rec[k].y);
+    const s16x2 c = as_s16(rec.y);
     D = f * c.xx + D;
     const s16x2 z = f * c.yy + D;
     u = __builtin_elementwise_max(u, z);
   }
+}
+
+template <int NW>
+__device__ __forceinline__ void step4_pk(const uint2 (&rec)[4], const uint32_t (&uA)[NW],
+                                         const uint32_t (&uB)[NW], s16x2& D, s16x2& u) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) step1_pk<NW>(rec[k], uA, uB, D, u);
+}
+
+// Free-job compaction for tiles of sibling children (instances above 32
+// jobs).  The children of a search round's 128-node tile belong to a few
+// sibling parents, so the jobs free in at least one of them (the tile's
+// union) are typically about half of n; a step of any other job has f = 0
+// in both halves of every lane and changes nothing.  The wave filters its pairs' steps by the union, in
+// chunks of kCompPairs(n4) pairs: every pair keeps exactly |union| steps, in
+// its Johnson order, so pair k of a chunk starts at k |union| in the wave's
+// LDS list, and the scan reads each step from LDS (one address for the
+// whole wave) instead of the shared table.
+constexpr int kCompPasses = 5;  // 64-step filter passes per chunk
+__device__ __forceinline__ int comp_pairs(int n4) { return max(1, 64 * kCompPasses / n4); }
+// uint2 records of a wave's compaction area (8 of read-ahead slack)
+__device__ __host__ __forceinline__ int comp_area(int n4) {
+  return (64 * kCompPasses / n4 > 1 ? 64 * kCompPasses / n4 : 1) * n4 + 8;
 }
 
 template <int MM, int NW, class Src>
@@ -552,6 +574,19 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
   const int G = A.groups;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tpb = (blockDim.x >> 6) / G;
+  // free-job compaction: sibling tiles of instances above 32 jobs (on 20
+  // jobs the filter and the per-pair LDS latency cost more than the ~40% of
+  // steps it removes; on 50 it saves a fifth of the solve)
+  constexpr bool kCompact = Src::kSiblings && NW >= 2;
+  // compaction only, after pl: the pairs' machines, then each wave's
+  // compacted step list
+  int2* s_pairs = reinterpret_cast<int2*>(pl + static_cast<size_t>(n) * 128 * tpb);
+  uint2* cs = reinterpret_cast<uint2*>(s_pairs + P) +
+              static_cast<size_t>(wave) * comp_area(T.n4);
+  if constexpr (kCompact) {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s_pairs[i] = T.pairs[i];
+    __syncthreads();
+  }
   const int tib = wave / G;
   const int grp = __builtin_amdgcn_readfirstlane(wave - tib * G);
   const int s_lo = (grp * P) / G, s_hi = ((grp + 1) * P) / G;
@@ -692,12 +727,109 @@ __global__ __launch_bounds__(256, (MM <= 20 ? 3 : 2)) void bound_kernel_pk(DevTa
     }
     s16x2 lb = as_s16(grp == 0 ? l1m : 0u);
     const s16x2 bias = as_s16(static_cast<uint32_t>(kBias16) * 0x10001u);
+    if constexpr (kCompact) {
+      // the tile's union of free jobs (valid nodes only), wave-uniform
+      uint32_t un[NW];
+      int c = 0;  // steps kept per pair
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        uint32_t x = (valid[0] ? uA[w] : 0u) | (valid[1] ? uB[w] : 0u);
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) x |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), d, 64));
+        un[w] = __builtin_amdgcn_readfirstlane(x);
+        c += __popc(un[w]);
+      }
+      const int n4 = T.n4, cpp = comp_pairs(n4);
+      const int kin0 = lane % n4;            // step index in its pair, first pass
+      const int r64 = 64 % n4;                // and its advance per pass
+      // a chunk's table loads (64 steps per pass, at most kCompPasses; the
+      // steps of a chunk stay inside its pairs, so nothing past the table is
+      // read); the next chunk's are in flight while this one is scanned
+      uint2 rec[kCompPasses];
+      auto load_chunk = [&](int sc0) {
+        const int t0 = sc0 * n4, t1 = (sc0 + min(cpp, s_hi - sc0)) * n4;
+#pragma unroll
+        for (int ps = 0; ps < kCompPasses; ++ps) {
+          const int tt = t0 + 64 * ps + lane;
+          rec[ps] = tt < t1 ? T.steps16[tt] : make_uint2(0u, 0u);
+        }
+      };
+      if (s_lo < s_hi) load_chunk(s_lo);
+      for (int sc = s_lo; sc < s_hi; sc += cpp) {
+        const int np = min(cpp, s_hi - sc);
+        const int t0 = sc * n4, t1 = (sc + np) * n4;
+        __builtin_amdgcn_wave_barrier();  // the previous chunk's reads first
+        // pair k's list starts at k c4 (c4 = c rounded up to four steps);
+        // the rounding slots hold zero steps, which change nothing
+        const int c4 = (c + 3) & ~3;
+        const float inv_c = 1.0f / static_cast<float>(max(c, 1));
+        for (int i = lane; i < np * c4; i += 64) cs[i] = make_uint2(0u, 0u);
+        __builtin_amdgcn_wave_barrier();
+        int carry = 0, kin = kin0;
+#pragma unroll
+        for (int ps = 0; ps < kCompPasses; ++ps) {
+          const int tt = t0 + 64 * ps + lane;
+          const uint32_t j = rec[ps].x;
+          uint32_t wd;
+          if constexpr (NW == 1) {
+            wd = un[0];
+          } else if constexpr (NW == 2) {
+            wd = (j >> 5) & 1u ? un[1] : un[0];
+          } else {
+            const uint32_t wi = (j >> 5) & 3u;
+            wd = wi == 0 ? un[0] : wi == 1 ? un[1] : wi == 2 ? un[2] : un[3];
+          }
+          const bool keep = tt < t1 && kin < n && ((wd >> (j & 31u)) & 1u);
+          const unsigned long long bal = __ballot(keep);
+          if (keep) {
+            // kept step K of the chunk: pair K / c, step K % c of that pair
+            // (K < 320, so the float quotient is exact after the floor)
+            const int K = carry + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+            const int pk = static_cast<int>((static_cast<float>(K) + 0.5f) * inv_c);
+            cs[pk * c4 + (K - pk * c)] = rec[ps];
+          }
+          carry += __popcll(bal);
+          kin += r64;
+          if (kin >= n4) kin -= n4;
+        }
+        if (sc + cpp < s_hi) load_chunk(sc + cpp);
+        __builtin_amdgcn_wave_barrier();  // the wave's LDS writes before its reads
+        // LDS only below (reads complete in order, so each wait is partial):
+        // the pairs' machines, and groups of four steps read one group ahead
+        const int2* pq = s_pairs + sc;
+        int2 abn = pq[0];
+        for (int pp = 0; pp < np; ++pp) {
+          const int a = __builtin_amdgcn_readfirstlane(abn.x);
+          const int b = __builtin_amdgcn_readfirstlane(abn.y);
+          abn = pq[min(pp + 1, np - 1)];
+          s16x2 D = as_s16(rp[a]) - bias;  // machine-a clocks, biased below zero
+          s16x2 u = as_s16(rp[b]);
+          const uint2* q = cs + pp * c4;
+          uint2 ga[4], gb[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) ga[i] = q[i];  // reads past the list stay in the slack
+          for (int k = 0; k < c4; k += 8) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gb[i] = q[k + 4 + i];
+            __builtin_amdgcn_sched_barrier(0);
+            step4_pk<NW>(ga, uA, uB, D, u);
+            if (k + 4 >= c4) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ga[i] = q[k + 8 + i];
+            __builtin_amdgcn_sched_barrier(0);
+            step4_pk<NW>(gb, uA, uB, D, u);
+          }
+          lb = __builtin_elementwise_max(lb, u + as_s16(ep[b]));
+        }
+      }
+    }
     // the pairs' step groups are contiguous in the table: walk them as one
     // stream; the read-ahead past the last group stays inside the table's
-    // 16 padding steps
+    // padding steps
     const int gpp = T.n4 >> 2;  // step groups per pair
     const int g_lo = s_lo * gpp, g_hi = s_hi * gpp;
-    if (g_lo < g_hi) {
+    if (!kCompact && g_lo < g_hi) {
       const uint2* gp = T.steps16 + static_cast<size_t>(g_lo) * 4;
       uint2 recA[4], recB[4];
       auto load_group = [&](const uint2* at, uint2 (&rc)[4]) {
@@ -772,8 +904,10 @@ hipError_t launch_pk(const DevTables& T, const Src& src, const BoundArgs& A, int
   int grid = (ntiles + tpb - 1) / tpb;
   if (grid > grid_cap) grid = grid_cap;
   if (grid < 1) grid = 1;
-  const size_t shmem = sizeof(int32_t) * (((MM * (T.n + 1) + 3) & ~3) + block) +
-                       static_cast<size_t>(T.n) * 128 * tpb;
+  size_t shmem = sizeof(int32_t) * (((MM * (T.n + 1) + 3) & ~3) + block) +
+                 static_cast<size_t>(T.n) * 128 * tpb;
+  if (Src::kSiblings && NW >= 2)
+    shmem += sizeof(int2) * T.P + sizeof(uint2) * comp_area(T.n4) * (block / 64);
   hipExtLaunchKernelGGL((bound_kernel_pk<MM, NW, Src>), dim3(grid), dim3(block), shmem, stream,
                         A.ev_start, A.ev_stop, 0, T, src, A);
   return hipGetLastError();

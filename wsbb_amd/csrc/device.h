@@ -134,6 +134,7 @@ __device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,
 // Node source: explicit position-major nodes.
 struct BatchSource {
   static constexpr bool kState = false;  // no parent state: heads/tails from the permutation
+  static constexpr bool kSiblings = false;  // consecutive nodes need not share free jobs
   const uint8_t* permT;  // [n][N]
   const int32_t* l1;
   const int32_t* l2;
@@ -155,6 +156,9 @@ struct BatchSource {
 // position-major parent array (decompose_beginend child semantics).
 struct ChildSource {
   static constexpr bool kState = true;
+  // consecutive children are a few sibling parents' children: a tile's free
+  // jobs are a small union (the two-node form scans only those)
+  static constexpr bool kSiblings = true;
   const uint8_t* ppermT;  // parents [n][PN]
   const int32_t* pl1;
   const int32_t* pl2;
@@ -230,6 +234,7 @@ struct ChildSource {
 // a round's children); results go to the listed child's slot.
 struct ListSource {
   static constexpr bool kState = true;
+  static constexpr bool kSiblings = false;
   ChildSource base;
   const int32_t* list;
   const int* count;
