@@ -568,15 +568,25 @@ def roofline(bt, peak, note):
             "algorithmic_bytes": bt["bytes"], "algorithmic_ops": bt["work"]}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr per finished section (the JSON line stays alone on stdout)."""
+    print(f"bench: {what} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned", "50x20"),
+    ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned",
+                                           "learned20", "50x20"),
                     default="all",
                     help="batched = the 50x20 launches only (--int16: two-node form); solve = "
-                         "ta011 time-to-optimum only; learned = the 20x20 learned comparison")
+                         "ta011 time-to-optimum only; learned = the 20x20 learned comparison "
+                         "and the online protocol; learned20 = the comparison only")
     ap.add_argument("--int16", action="store_true", help="batched: the two-node int16 kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-autotune", action="store_true")
@@ -609,6 +619,9 @@ def main():
                           "learned_online": bench_learned_online(wsbb_amd,
                                                                  runs=args.online_runs)}),
               flush=True)
+        return
+    if args.workload == "learned20":
+        print(json.dumps({"learned_20x20": bench_learned(wsbb_amd)}), flush=True)
         return
     if args.workload == "50x20":
         print(json.dumps(bench_50x20(wsbb_amd, host, cpu=not args.no_cpu)), flush=True)
@@ -646,8 +659,10 @@ def main():
         "roofline": rf,
         "autotune": bt["tuned"],
     }
+    progress("ta051 batched int32")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_batched(bt, host)
+        progress("CPU baseline sample")
     b16 = bench_batched(wsbb_amd, args.steps, args.warmup, int16=True,
                         autotune=not args.no_autotune)
     r16 = roofline(b16, 2 * PEAK_VALU_TOPS, "bound_kernel_pk<20, 2, BatchSource> (two nodes "
@@ -658,13 +673,20 @@ def main():
         "ms_per_launch": b16["wall_ms"], "roofline": r16,
         "same_values_as_int32": bool(np.array_equal(b16["out"], bt["out"])),
         "applies": "instances whose path bound stays below 2^14 (every Taillard instance)"}
+    progress("ta051 batched int16")
     line["config1"] = bench_config1(wsbb_amd, host, cpu=not args.no_cpu)
+    progress("ta001-ta010 solves")
     line["config2"] = bench_config2(wsbb_amd, host, cpu=not args.no_cpu)
+    progress("ta011 solves")
     if not args.no_learned:
         line["learned_20x20"] = bench_learned(wsbb_amd)
+        progress("ta021 learned comparison")
         line["learned_online"] = bench_learned_online(wsbb_amd, runs=args.online_runs)
+        progress("ta021-ta030 online learning")
         line["mixed_batch"] = bench_mixed(wsbb_amd, host=None if args.no_cpu else host)
+        progress("mixed batch")
         line["solve_50x20"] = bench_50x20(wsbb_amd, host, cpu=not args.no_cpu)
+        progress("50x20 solves")
     print(json.dumps(line), flush=True)
 
 
