@@ -559,6 +559,24 @@ def cpu_batched(bt, host):
             "parity": bool(np.array_equal(ref, bt["out"][:S1]))}
 
 
+def valu_peak(wsbb_amd, iters=20000):
+    """Measured integer VALU throughput of the probe kernel (eight waves per
+    SIMD, independent chains of one instruction), T lane-ops/s per kind."""
+    out = {}
+    with wsbb_amd.Engine(wsbb_amd.taillard(1)) as eng:
+        nbytes = eng.num_cus * 8 * 256 * 4
+        d = eng.malloc(nbytes)
+        for kind, name in ((0, "v_add_u32"), (1, "v_mad_u32_u24"), (2, "v_max_i32")):
+            eng.valu_probe(kind, 16, d, nbytes)
+            eng.synchronize()
+            eng.timer_start()
+            ops = eng.valu_probe(kind, iters, d, nbytes)
+            ms = eng.timer_stop()
+            out[name] = ops / (ms / 1e3) / 1e12
+        eng.free(d)
+    return out
+
+
 def roofline(bt, peak, note):
     ach = bt["work"] / (bt["ms"] / 1e3) / 1e12
     return {"bound": "valu", "kernel": note, "achieved": ach, "peak": peak, "unit": "Tint-op/s",
@@ -582,7 +600,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("all", "batched", "solve", "copy", "learned",
-                                           "learned20", "50x20"),
+                                           "learned20", "50x20", "peak"),
                     default="all",
                     help="batched = the 50x20 launches only (--int16: two-node form); solve = "
                          "ta011 time-to-optimum only; learned = the 20x20 learned comparison "
@@ -620,6 +638,9 @@ def main():
                                                                  runs=args.online_runs)}),
               flush=True)
         return
+    if args.workload == "peak":
+        print(json.dumps({"workload": "peak", "valu_tops": valu_peak(wsbb_amd)}), flush=True)
+        return
     if args.workload == "learned20":
         print(json.dumps({"learned_20x20": bench_learned(wsbb_amd)}), flush=True)
         return
@@ -639,6 +660,10 @@ def main():
 
     rf = roofline(bt, PEAK_VALU_TOPS, "bound_kernel<20, 2, 0, 256, BatchSource> (int32)")
     rf["traffic"] = measured_traffic("batched_int32")
+    # the derived peak against the probe kernel's measured rates on this GPU
+    rf["peak_check"] = {"measured_tlaneops": valu_peak(wsbb_amd),
+                        "derived": "256 CU x 64 lanes x 2.4 GHz = 39.3 T int32 lane-ops/s",
+                        "scan_instructions": "v_bfe_u32, v_mad_i32_i24, v_mad_u32_u24, v_max_i32"}
     line = {
         "metric": "LB2 bounded nodes/s + time-to-proven-optimum, 1 MI355X vs host CPU",
         "value": bt["N"] / (bt["wall_ms"] / 1e3),

@@ -148,6 +148,15 @@ int wsbb_synchronize(wsbb_ctx *ctx);
  * reference: it calibrates hardware byte counters against a known size. */
 int wsbb_copy_device(wsbb_ctx *ctx, void *d_dst, const void *d_src,
                      size_t bytes);
+/* Integer VALU throughput probe, asynchronous on the context's stream: eight
+ * blocks of 256 threads per CU, each thread running 16 x iters independent
+ * instructions of one kind (0 v_add_u32, 1 v_mad_u32_u24, 2 v_max_i32, the
+ * Johnson scan's operations); *lane_ops receives the lane-operations the
+ * launch executes, d_out (num_cus x 8192 bytes) the threads' results.  A
+ * measurement helper with no counterpart in the reference: it checks the
+ * int32 lane-op peak the bound kernels are measured against. */
+int wsbb_valu_probe(wsbb_ctx *ctx, int kind, int iters, void *d_out,
+                    size_t out_bytes, double *lane_ops);
 /* HIP events recorded on the context's stream around a timed region */
 int wsbb_timer_start(wsbb_ctx *ctx);
 int wsbb_timer_stop(wsbb_ctx *ctx, float *ms);

@@ -103,3 +103,22 @@ def test_budget_and_overflow_codes_then_a_full_solve():
         eng.set_pool_capacity(1 << 22)
         res = eng.solve(pool_size=4096, max_batches=10)
         assert res["proven"] and res["best"] == 1582
+
+
+def test_valu_probe_counts_its_lane_operations():
+    """The integer VALU probe (the check of the derived int32 lane-op peak)
+    counts 16 instructions per thread and iteration over eight 256-thread
+    blocks per CU, refuses bad kinds and a short buffer, and a run of every
+    kind finishes with the context still usable."""
+    with wsbb_amd.Engine(oracle.taillard(1)[3]) as eng:
+        nbytes = eng.num_cus * 8 * 256 * 4
+        d = eng.malloc(nbytes)
+        ops = ctypes.c_double()
+        expect_einval(L.wsbb_valu_probe(eng.h, 3, 10, d, nbytes, ctypes.byref(ops)), "kind")
+        expect_einval(L.wsbb_valu_probe(eng.h, 0, 10, d, nbytes - 4, ctypes.byref(ops)), "buffer")
+        for kind in range(3):
+            assert eng.valu_probe(kind, 10, d, nbytes) == 16 * 10 * eng.num_cus * 8 * 256
+        eng.synchronize()
+        eng.free(d)
+        permT, l1, l2 = wsbb_amd.random_nodes(eng.n, 256, 3)
+        assert np.array_equal(eng.bound(permT, l1, l2), oracle.Oracle(oracle.taillard(1)[3]).bound_batch(permT, l1, l2))

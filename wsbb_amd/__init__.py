@@ -106,6 +106,8 @@ def _load():
                                _u8p, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int32, _i64p,
                                _i32p, _i32p, _i32p, _i64p], ctypes.c_int),
         "wsbb_copy_device": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "wsbb_valu_probe": ([_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t,
+                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "wsbb_selector_init": ([_vp, ctypes.c_uint64], ctypes.c_int),
         "wsbb_selector_params": ([_vp, ctypes.c_int, _f32p, _f32p, _f32p, _f32p], ctypes.c_int),
         "wsbb_selector_config": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -305,6 +307,13 @@ class Engine:
 
     def copy_device(self, d_dst, d_src, nbytes):
         _check(lib.wsbb_copy_device(self.h, d_dst, d_src, nbytes), "copy_device")
+
+    def valu_probe(self, kind, iters, d_out, nbytes):
+        """Launches the integer VALU probe; returns the lane-ops it executes."""
+        ops = ctypes.c_double()
+        _check(lib.wsbb_valu_probe(self.h, kind, iters, d_out, nbytes, ctypes.byref(ops)),
+               "valu_probe")
+        return ops.value
 
     def synchronize(self):
         _check(lib.wsbb_synchronize(self.h), "synchronize")

@@ -685,9 +685,67 @@ __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst,
        i += size_t(gridDim.x) * blockDim.x)
     dst[i] = src[i];
 }
+// Integer VALU throughput probe: eight independent chains of one VALU
+// instruction per thread, written as inline asm so the count per iteration
+// is exact (kind 0 v_add_u32, 1 v_mad_u32_u24, 2 v_max_i32: the operations of
+// the Johnson scan).  Each thread stores its chains' XOR so nothing is
+// dead.  One lane-op per instruction and lane.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t* __restrict__ out, int iters) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a0 = t, a1 = t ^ 1u, a2 = t ^ 2u, a3 = t ^ 3u, a4 = t ^ 4u, a5 = t ^ 5u, a6 = t ^ 6u,
+           a7 = t ^ 7u;
+  const uint32_t b = (t & 7u) + 1u, c = (t >> 3) & 7u;
+  for (int i = 0; i < iters; ++i) {
+#define WSBB_P8(INS)                                             \
+  asm volatile(INS : "+v"(a0) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a1) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a2) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a3) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a4) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a5) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a6) : "v"(b), "v"(c));                  \
+  asm volatile(INS : "+v"(a7) : "v"(b), "v"(c));
+    if constexpr (KIND == 0) {
+      WSBB_P8("v_add_u32 %0, %0, %1")
+      WSBB_P8("v_add_u32 %0, %0, %2")
+    } else if constexpr (KIND == 1) {
+      WSBB_P8("v_mad_u32_u24 %0, %1, %2, %0")
+      WSBB_P8("v_mad_u32_u24 %0, %2, %1, %0")
+    } else {
+      WSBB_P8("v_max_i32 %0, %0, %1")
+      WSBB_P8("v_max_i32 %0, %2, %0")
+    }
+#undef WSBB_P8
+  }
+  out[t] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
 }  // namespace
 
 extern "C" {
+
+int wsbb_valu_probe(wsbb_ctx* ctx, int kind, int iters, void* d_out, size_t out_bytes,
+                    double* lane_ops) {
+  WSBB_CHECK_CTX(ctx);
+  if (kind < 0 || kind > 2 || iters < 1 || lane_ops == nullptr || d_out == nullptr) {
+    set_error("valu probe: kind 0..2, iters >= 1, output buffer and lane_ops required");
+    return WSBB_EINVAL;
+  }
+  // eight blocks of four waves per CU: eight waves per SIMD
+  const int grid = ctx->num_cus * 8, block = 256;
+  if (out_bytes < sizeof(uint32_t) * size_t(grid) * block) {
+    set_error("valu probe: the output buffer needs " +
+              std::to_string(sizeof(uint32_t) * size_t(grid) * block) + " bytes");
+    return WSBB_EINVAL;
+  }
+  uint32_t* out = static_cast<uint32_t*>(d_out);
+  if (kind == 0) hipLaunchKernelGGL(k_valu_probe<0>, dim3(grid), dim3(block), 0, ctx->stream, out, iters);
+  if (kind == 1) hipLaunchKernelGGL(k_valu_probe<1>, dim3(grid), dim3(block), 0, ctx->stream, out, iters);
+  if (kind == 2) hipLaunchKernelGGL(k_valu_probe<2>, dim3(grid), dim3(block), 0, ctx->stream, out, iters);
+  WSBB_HIP(hipGetLastError(), "valu probe launch");
+  *lane_ops = 16.0 * double(iters) * grid * block;
+  return WSBB_OK;
+}
 
 int wsbb_copy_device(wsbb_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
   WSBB_CHECK_CTX(ctx);
