@@ -141,6 +141,19 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   // tiles, staged once per tile so the head/tail recurrences read LDS
   // instead of waiting on one global load per position
   uint8_t* pl = reinterpret_cast<uint8_t*>(part + blockDim.x);
+  const int G = A.groups;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tpb = (blockDim.x >> 6) / G;  // 64-node tiles per block
+  const int tib = wave / G;
+  // the live count and (children of a round) the first tile's descriptors
+  // load with the staging below instead of after its barrier: two memory
+  // round trips fewer in front of a search round's bound
+  const int N = (A.Ndev != nullptr) ? *A.Ndev : A.N;
+  int key0 = 0;
+  if constexpr (Src::kPrefetch) {
+    const int node0 = (blockIdx.x * tpb + tib) * 64 + lane;
+    if (node0 < A.N) key0 = src.key(node0);
+  }
   stage_times<MM>(T, sp);
   for (int i = threadIdx.x; i < 3 * MM; i += blockDim.x) {
     const int c = i / MM, k = i - c * MM;
@@ -149,16 +162,11 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
   }
   __syncthreads();
 
-  const int G = A.groups;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tpb = (blockDim.x >> 6) / G;  // 64-node tiles per block
-  const int tib = wave / G;
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // pair loop, its table loads and the register-array indices scalar
   const int grp = __builtin_amdgcn_readfirstlane(wave - tib * G);
   const int s_lo = (A.kind == 0) ? (grp * P) / G : 0;
   const int s_hi = (A.kind == 0) ? ((grp + 1) * P) / G : 0;
-  const int N = (A.Ndev != nullptr) ? *A.Ndev : A.N;
   const size_t ld = A.ld > 0 ? static_cast<size_t>(A.ld) : static_cast<size_t>(N);
   const int ntiles = (N + 63) >> 6;
   const int nbt = (ntiles + tpb - 1) / tpb;
@@ -179,7 +187,16 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
     const uint8_t* col = nullptr;
     // the live count (device-side in a search round) bounds the last tile;
     // a source's own limit is its capacity
-    const bool valid = node < N && src.load(node, l1, l2, col, stride, sw0, sw1);
+    bool valid;
+    int key = 0;  // descriptor of the child whose state this lane builds
+    if constexpr (Src::kPrefetch) {
+      valid = node < N;
+      if (bt == static_cast<int>(blockIdx.x)) key = key0;
+      if (bt != static_cast<int>(blockIdx.x) || !valid) key = src.key(valid ? node : N - 1);
+      if (valid) src.load_key(key, l1, l2, col, stride, sw0, sw1);
+    } else {
+      valid = node < N && src.load(node, l1, l2, col, stride, sw0, sw1);
+    }
     if (!valid) {
       l1 = -1;
       l2 = n;
@@ -231,7 +248,10 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
       // path without a zero fill held in registers beside them
       if constexpr (Src::kState) {
         int q[MM], sc[MM];
-        src.template state<MM, NW>(min(node, N - 1), sp, r, q, sm, sc);
+        if constexpr (Src::kPrefetch)
+          src.template state_key<MM, NW>(key, sp, r, q, sm, sc);
+        else
+          src.template state<MM, NW>(min(node, N - 1), sp, r, q, sm, sc);
         int tot[MM], q0[MM];
         job_row<MM>(sp, n + 2, tot);
         job_row<MM>(sp, n + 1, q0);

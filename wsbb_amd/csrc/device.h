@@ -135,6 +135,7 @@ __device__ __forceinline__ int perm_at(const uint8_t* col, int stride, int sw0,
 struct BatchSource {
   static constexpr bool kState = false;  // no parent state: heads/tails from the permutation
   static constexpr bool kSiblings = false;  // consecutive nodes need not share free jobs
+  static constexpr bool kPrefetch = false;
   const uint8_t* permT;  // [n][N]
   const int32_t* l1;
   const int32_t* l2;
@@ -159,6 +160,10 @@ struct ChildSource {
   // consecutive children are a few sibling parents' children: a tile's free
   // jobs are a small union (the two-node form scans only those)
   static constexpr bool kSiblings = true;
+  // a child's descriptor can be read before anything else is known (any k
+  // below the capacity N is inside the array), so the bound kernel issues
+  // its first tile's descriptors with its staging loads
+  static constexpr bool kPrefetch = true;
   const uint8_t* ppermT;  // parents [n][PN]
   const int32_t* pl1;
   const int32_t* pl2;
@@ -174,10 +179,16 @@ struct ChildSource {
   __device__ __forceinline__ bool has_state() const { return hr != nullptr; }
   // heads, tails, scheduled mask and scheduled work of child k from its
   // parent's state; sp holds the processing times job-major in LDS
+  __device__ __forceinline__ int key(int k) const { return desc[k]; }
   template <int MM, int NW>
   __device__ __forceinline__ void state(int k, const int32_t* sp, int (&r)[MM], int (&q)[MM],
                                         uint32_t (&sm)[NW], int (&sc)[MM]) const {
-    const int d = desc[k];
+    state_key<MM, NW>(desc[k], sp, r, q, sm, sc);
+  }
+  // the same from the child's descriptor d
+  template <int MM, int NW>
+  __device__ __forceinline__ void state_key(int d, const int32_t* sp, int (&r)[MM], int (&q)[MM],
+                                            uint32_t (&sm)[NW], int (&sc)[MM]) const {
     const int par = d >> 9, side = (d >> 8) & 1, j = d & 255;
     const int pa = pl1[par], pb = pl2[par];
 #pragma unroll
@@ -214,7 +225,12 @@ struct ChildSource {
   __device__ __forceinline__ bool load(int k, int& a, int& b, const uint8_t*& col,
                                        int& stride, int& sw0, int& sw1) const {
     if (k >= N) return false;
-    const int d = desc[k];
+    load_key(desc[k], a, b, col, stride, sw0, sw1);
+    return true;
+  }
+  // limits and permutation view of the child with descriptor d
+  __device__ __forceinline__ void load_key(int d, int& a, int& b, const uint8_t*& col,
+                                           int& stride, int& sw0, int& sw1) const {
     const int par = d >> 9, side = (d >> 8) & 1, j = d & 255;
     const int pa = pl1[par], pb = pl2[par];
     // limites_set / permutation_set (bb_problem_calculator.cpp:40-48, 66-74)
@@ -225,7 +241,6 @@ struct ChildSource {
     sw1 = j;
     col = ppermT + par;
     stride = PN;
-    return true;
   }
   __device__ __forceinline__ int slot(int k) const { return k; }
 };
@@ -235,6 +250,7 @@ struct ChildSource {
 struct ListSource {
   static constexpr bool kState = true;
   static constexpr bool kSiblings = false;
+  static constexpr bool kPrefetch = false;
   ChildSource base;
   const int32_t* list;
   const int* count;
