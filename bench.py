@@ -613,8 +613,8 @@ def main():
     ap.add_argument("--online-runs", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build targets a single MI355X "
-              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs on a single MI355X "
+              "(sharding across GPUs is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import wsbb_amd
