@@ -46,12 +46,12 @@ def test_ta011_config_every_child_bound_matches_oracle():
     assert total == st["bounded"]
 
 
-@pytest.mark.parametrize("tid,pool,nb,budget", [(11, 4096, 1, 0), (21, 65536, 4, 300_000),
-                                                (51, 4096, 10, 40_000)])
+@pytest.mark.parametrize("tid,pool,nb,budget", [(11, 4096, 10, 0), (21, 65536, 4, 30_000),
+                                                (51, 4096, 10, 6_000)])
 def test_two_node_search_rounds_match_oracle(tid, pool, nb, budget):
     """Search rounds forced through the two-node int16 form: every child of
     every round against the oracle's bornes_calculer, on a 20x10 solve to the
-    optimum, a budgeted 20x20 solve with the 2^18-child rounds that select
+    optimum (ten pools per round), a budgeted 20x20 solve with the 2^18-child rounds that select
     this form by default, and a budgeted 50x20 solve, where each 128-child
     tile scans only the union of its children's free jobs (two mask words
     per node)."""
