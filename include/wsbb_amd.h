@@ -281,8 +281,9 @@ int wsbb_solve_resume(wsbb_ctx *ctx, int kind, int pool_size, int max_batches,
  * wsbb_pool_seed_root pushes the root (tree::add_root, tree.cpp:170-176). */
 int wsbb_pool_open(wsbb_ctx *ctx, int kind, int pool_size, int max_batches);
 int wsbb_pool_seed_root(wsbb_ctx *ctx);
-/* WSBB_EINVAL for a node with fewer than two free jobs or a column that is
- * not a permutation; WSBB_EOVERFLOW beyond the pool capacity. */
+/* WSBB_EINVAL for a node with fewer than two free jobs, a negative bound or
+ * a column that is not a permutation; WSBB_EOVERFLOW beyond the pool
+ * capacity. */
 int wsbb_pool_push(wsbb_ctx *ctx, const uint8_t *permT, const int32_t *l1,
                    const int32_t *l2, const int32_t *cost, int count);
 /* Takes up to max nodes off the pool (*count of them): from the top, the

@@ -153,7 +153,8 @@ def test_pool_overflow_stops_cleanly_and_resumes_exactly():
 
 def test_resume_rejects_nodes_the_device_never_makes():
     """wsbb_solve_resume refuses a pool node with fewer than two free jobs,
-    a column that is not a permutation, and a bad incumbent permutation."""
+    a negative bound, a column that is not a permutation, and a bad
+    incumbent permutation; a bound beyond any incumbent still resumes."""
     p = oracle.taillard(1)[3]
     n = p.shape[1]
     base = dict(inc=2**31 - 1, stats=np.zeros(wsbb_amd.NSTATS, np.int64),
@@ -167,10 +168,14 @@ def test_resume_rejects_nodes_the_device_never_makes():
     dup[1, 0] = dup[0, 0]
     bad_perm = dict(base, permT=dup)
     bad_inc = dict(base, best_perm=np.full(n, 0, np.int32))
+    bad_cost = dict(base, cost=np.array([-5], np.int32))
     with wsbb_amd.Engine(p) as eng:
-        for ck in (bad_leaf, bad_one, bad_perm, bad_inc):
+        for ck in (bad_leaf, bad_one, bad_perm, bad_inc, bad_cost):
             with pytest.raises(wsbb_amd.WsbbError) as ei:
                 eng.resume(ck, pool_size=4096)
             assert ei.value.code == -1
         res = eng.resume(base, pool_size=4096)
+        assert res["proven"] and res["best"] == 1278
+        # a root carrying a huge bound is still taken without an incumbent
+        res = eng.resume(dict(base, cost=np.array([2**30], np.int32)), pool_size=4096)
     assert res["proven"] and res["best"] == 1278

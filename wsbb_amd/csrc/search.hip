@@ -52,6 +52,12 @@ constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
 constexpr int kSelWin = 16384;  // candidates per select window after the first
 constexpr int kSelWin0 = 4096;  // candidates in the first window
 constexpr int kSettleK = 16;   // consecutive parents per settle thread
+// largest cost a select key holds: bounds stay below 2^19, and a larger
+// host-given cost compares with every incumbent as this one does
+constexpr int kKeyCost = (1 << 23) - 1;
+__host__ __device__ inline int select_key(int l1, int l2, int cost, int n) {
+  return ((cost < kKeyCost ? cost : kKeyCost) << 8) | ((l1 + 1) + (n - l2));
+}
 
 struct Ctrl {
   int status;
@@ -79,6 +85,8 @@ struct SArgs {
   int* s_l1;
   int* s_l2;
   int* s_cost;
+  int* s_key;       // [CAP] what select reads of a node, one word:
+                    // min(cost, kKeyCost) << 8 | depth
   uint8_t* p_perm;  // [n][PCAP]
   int* p_l1;
   int* p_l2;
@@ -254,26 +262,22 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
   // coalesced loads: candidate i = t + 1024 k is the i-th from the top; its
   // reservation w (push_back1) and children c share one LDS word (both
   // at most 2n)
-  // every load is issued before the first use (indices past the window are
-  // clamped to a valid slot and their words zeroed), in groups of eight
+  // one key word per candidate (cost and depth); every load is issued
+  // before the first use (indices past the window are clamped to a valid
+  // slot and their words zeroed)
+  {
+    int key[KPER];
 #pragma unroll
-  for (int k0 = 0; k0 < KPER; k0 += 8) {
-    int a1[8], a2[8], co[8];
-#pragma unroll
-    for (int k = 0; k < 8 && k0 + k < KPER; ++k) {
-      const int i = tid + 1024 * (k0 + k);
-      const int si = top - 1 - (st.consumed + min(i, win - 1));
-      a1[k] = S.s_l1[si];
-      a2[k] = S.s_l2[si];
-      co[k] = S.s_cost[si];
+    for (int k = 0; k < KPER; ++k) {
+      const int i = tid + 1024 * k;
+      key[k] = S.s_key[top - 1 - (st.consumed + min(i, win - 1))];
     }
 #pragma unroll
-    for (int k = 0; k < 8 && k0 + k < KPER; ++k) {
-      const int i = tid + 1024 * (k0 + k);
-      const int depth = (a1[k] + 1) + (n - a2[k]);
-      const int u = n - depth;
-      const int w = (co[k] < inc) ? 2 * u : 0;  // reservation of push_back1
-      const int c = (depth == n - 2) ? 2 : 2 * u;  // children actually generated
+    for (int k = 0; k < KPER; ++k) {
+      const int i = tid + 1024 * k;
+      const int depth = key[k] & 255, u = n - depth;
+      const int w = ((key[k] >> 8) < inc) ? 2 * u : 0;  // reservation of push_back1
+      const int c = (depth == n - 2) ? 2 : 2 * u;       // children actually generated
       s_p[sel_pad(i)] = i < win ? (w | (c << 16)) : 0;
     }
   }
@@ -958,6 +962,7 @@ __global__ __launch_bounds__(256) void k_push(SArgs S) {
           S.s_l1[slot] = l1;
           S.s_l2[slot] = l2;
           S.s_cost[slot] = cst[t];
+          S.s_key[slot] = select_key(l1, l2, cst[t], n);
         }
       }
     }
@@ -1128,6 +1133,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_sl1 = take(4 * size_t(cap_nodes));
   const size_t o_sl2 = take(4 * size_t(cap_nodes));
   const size_t o_scost = take(4 * size_t(cap_nodes));
+  const size_t o_skey = take(4 * size_t(cap_nodes));
   const size_t o_pperm = take(size_t(n) * PCAP);
   // parent arrays: PCAP + 1 entries plus room for the settle kernel's
   // 16-wide vector accesses past the last parent
@@ -1171,6 +1177,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.s_l1 = S->mem.as<int>(o_sl1);
   A.s_l2 = S->mem.as<int>(o_sl2);
   A.s_cost = S->mem.as<int>(o_scost);
+  A.s_key = S->mem.as<int>(o_skey);
   A.p_perm = S->mem.as<uint8_t>(o_pperm);
   int** pints[13] = {&A.p_l1,      &A.p_l2,      &A.p_cost,    &A.p_sel,
                      &A.p_off,     &A.p_side,    &A.p_nch,     &A.p_leafmin, &A.p_leafmax,
@@ -1450,7 +1457,7 @@ static int push_nodes(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1, co
   const int n = ctx->T.n;
   for (int k = 0; k < count; ++k) {
     const int a = l1[k], b = l2[k];
-    bool ok = a >= -1 && b <= n && b - a >= 3;
+    bool ok = a >= -1 && b <= n && b - a >= 3 && cost[k] >= 0;
     uint64_t seen[2] = {0, 0};
     for (int i = 0; ok && i < n; ++i) {
       const int j = permT[static_cast<size_t>(i) * count + k];
@@ -1459,7 +1466,7 @@ static int push_nodes(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1, co
     }
     if (!ok) {
       set_error("pool node " + std::to_string(k) +
-                " has fewer than two free jobs or is not a permutation of 0..n-1");
+                " has fewer than two free jobs, a negative bound or is not a permutation of 0..n-1");
       return WSBB_EINVAL;
     }
   }
@@ -1476,11 +1483,19 @@ static int push_nodes(wsbb_ctx* ctx, const uint8_t* permT, const int32_t* l1, co
   const SArgs& A = S->sess;
   hipError_t e = hipMemcpy2DAsync(A.s_perm + top, S->CAP, permT, count, count, n,
                                   hipMemcpyHostToDevice, ctx->stream);
+  // select's key words, built here (the stream is synchronised before the
+  // host vector goes out of scope)
+  std::vector<int32_t> key(count);
+  for (int k = 0; k < count; ++k) key[k] = select_key(l1[k], l2[k], cost[k], n);
   const std::pair<int*, const int32_t*> rows[] = {{A.s_l1, l1}, {A.s_l2, l2}, {A.s_cost, cost}};
   for (const auto& r : rows)
     if (e == hipSuccess)
       e = hipMemcpyAsync(r.first + top, r.second, 4 * size_t(count), hipMemcpyHostToDevice,
                          ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(A.s_key + top, key.data(), 4 * size_t(count), hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "pool push");
   h.stack_size = top + count;
   if (h.status == kDone) h.status = kRunning;
@@ -1524,8 +1539,8 @@ static int pop_nodes(wsbb_ctx* ctx, int max, int from_bottom, uint8_t* permT, in
         e = hipMemcpy2DAsync(A.s_perm, S->CAP, tp, rest, rest, n, hipMemcpyDeviceToDevice,
                              ctx->stream);
       int* ti = reinterpret_cast<int*>(tp + static_cast<size_t>(n) * rest);
-      int* cols[] = {A.s_l1, A.s_l2, A.s_cost};
-      for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+      int* cols[] = {A.s_l1, A.s_l2, A.s_cost, A.s_key};
+      for (int c = 0; c < 4 && e == hipSuccess; ++c) {
         e = hipMemcpyAsync(ti, cols[c] + k, 4 * rest, hipMemcpyDeviceToDevice, ctx->stream);
         if (e == hipSuccess)
           e = hipMemcpyAsync(cols[c], ti, 4 * rest, hipMemcpyDeviceToDevice, ctx->stream);
