@@ -46,20 +46,23 @@ def test_ta011_config_every_child_bound_matches_oracle():
     assert total == st["bounded"]
 
 
-@pytest.mark.parametrize("tid,pool,nb,budget", [(11, 4096, 10, 0), (21, 65536, 4, 30_000),
-                                                (51, 4096, 10, 6_000)])
-def test_two_node_search_rounds_match_oracle(tid, pool, nb, budget):
-    """Search rounds forced through the two-node int16 form: every child of
-    every round against the oracle's bornes_calculer, on a 20x10 solve to the
-    optimum (ten pools per round), a budgeted 20x20 solve with the 2^18-child rounds that select
-    this form by default, and a budgeted 50x20 solve, where each 128-child
-    tile scans only the union of its children's free jobs (two mask words
-    per node)."""
+@pytest.mark.parametrize("tid,pool,nb,budget,int16", [(11, 4096, 10, 0, 2), (21, 65536, 4, 30_000, 2),
+                                                       (51, 4096, 10, 6_000, 2),
+                                                       (81, 4096, 4, 1_500, 2),
+                                                       (81, 4096, 4, 1_500, 0)])
+def test_two_node_search_rounds_match_oracle(tid, pool, nb, budget, int16):
+    """Search rounds forced through the two-node int16 form (int16 = 2), or
+    through the int32 kernel (0): every child of every round against the
+    oracle's bornes_calculer, on a 20x10 solve to the optimum (ten pools per
+    round), a budgeted 20x20 solve with the 2^18-child rounds that select the
+    two-node form by default, and budgeted 50x20 and 100x20 solves, where
+    each 128-child tile of the two-node form scans only the union of its
+    children's free jobs (two and four mask words per node)."""
     p = oracle.taillard(tid)[3]
     n = p.shape[1]
     o = oracle.Oracle(p)
     best, st, _ = o.solve_batched(pool_size=pool, max_batches=nb, budget=budget)
-    res, rounds = traced_solve(p, pool, nb, int16=2, budget=budget)
+    res, rounds = traced_solve(p, pool, nb, int16=int16, budget=budget)
     # no leaf reached before the budget: the oracle says -1, the device INT32_MAX
     assert res["best"] == (best if best >= 0 else 2**31 - 1)
     assert res["stats"]["bounded"] == st["bounded"]
