@@ -77,6 +77,30 @@ def test_two_node_search_rounds_match_oracle(tid, pool, nb, budget, int16):
     assert total == st["bounded"]
 
 
+@pytest.mark.parametrize("tid,budget", [(11, 0), (51, 3_000)])
+def test_lb1_search_rounds_match_oracle(tid, budget):
+    """Search rounds with the one-machine bound: every child of every round
+    against the oracle's LB1, on a 20x10 solve to the optimum and a budgeted
+    50x20 solve (two mask words per node), with the oracle explorer's
+    counters."""
+    p = oracle.taillard(tid)[3]
+    n = p.shape[1]
+    o = oracle.Oracle(p)
+    best, st, _ = o.solve_batched(kind=oracle.LB1, pool_size=4096, max_batches=10, budget=budget)
+    res, rounds = traced_solve(p, 4096, 10, kind=wsbb_amd.LB1, budget=budget)
+    assert res["best"] == (best if best >= 0 else 2**31 - 1)
+    for k in ("decomposed", "bounded"):
+        assert res["stats"][k] == st[k], k
+    total = 0
+    for rnd in rounds:
+        permT, l1, l2 = rebuild_children(rnd, n)
+        ref = o.bound_batch(permT, l1, l2, kind=oracle.LB1, threads=8)
+        bad = np.flatnonzero(ref != rnd["bound"])
+        assert bad.size == 0, (rnd["round"], bad[:5], ref[bad[:5]], rnd["bound"][bad[:5]])
+        total += l1.shape[0]
+    assert total == st["bounded"]
+
+
 @pytest.mark.parametrize("tid,pool,nb", [(1, 4096, 1), (3, 300, 3), (11, 4096, 10)])
 def test_parents_follow_the_oracle_pop_sequence(tid, pool, nb):
     """The parents each device round takes, in pop order, are exactly the
