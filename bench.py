@@ -259,7 +259,7 @@ def bench_learned(wsbb_amd):
     logged by a plain solve of the same instance.  Hit rate: share of the
     children of 4096 random parents whose tile-subset bound equals full LB2."""
     p = wsbb_amd.taillard(21)
-    pool, nb, topk = 65536, 4, 16
+    pool, nb, topk = 65536, 8, 16  # the fastest plain shape measured on ta021-ta024
     with wsbb_amd.Engine(p) as eng:
         eng.selector_init(seed=1)
         eng.selector_config(topk=topk, margin=-1, parents=True)
@@ -289,7 +289,7 @@ def bench_learned(wsbb_amd):
         sub, _ = eng.bound_tiles(cpermT, cl1, cl2, tsel, margin=-1)
     ps, ls = plain["stats"], learned["stats"]
     return {
-        "workload": "ta021 20x20 solved to proven optimum, 65536-child pools, 4 sub-batches",
+        "workload": "ta021 20x20 solved to proven optimum, 65536-child pools, 8 sub-batches",
         "selector": f"MLP 100-128-190 on each round's parents, BF16 weights / FP32 "
                     f"accumulation, top-{topk} pairs per 64-child tile from the parents' "
                     "probabilities weighted by their children in the tile, trained on records "
@@ -348,7 +348,7 @@ def bench_learned_online(wsbb_amd, ids=tuple(range(21, 31)), runs=3, warm=2_000_
     """Online learning on one instance, all costs counted (online_run), set
     against a plain solve of the same instance from scratch; `runs` of each
     per instance, alternating, with mean and spread of the times."""
-    pool, nb, cap = 65536, 4, 1 << 17
+    pool, nb, cap = 65536, 8, 1 << 17  # the fastest plain shape measured on ta021-ta024
     out = []
     for tid in ids:
         plain_t, online_t, train_t = [], [], []
