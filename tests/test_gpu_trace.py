@@ -101,18 +101,21 @@ def test_lb1_search_rounds_match_oracle(tid, budget):
     assert total == st["bounded"]
 
 
-@pytest.mark.parametrize("tid,pool,nb", [(1, 4096, 1), (3, 300, 3), (11, 4096, 10)])
-def test_parents_follow_the_oracle_pop_sequence(tid, pool, nb):
+@pytest.mark.parametrize("tid,pool,nb,budget", [(1, 4096, 1, 0), (3, 300, 3, 0), (11, 4096, 10, 0),
+                                                (21, 65536, 8, 60_000), (51, 65536, 4, 8_000)])
+def test_parents_follow_the_oracle_pop_sequence(tid, pool, nb, budget):
     """The parents each device round takes, in pop order, are exactly the
     oracle explorer's (whose order is pinned to the reference's rank-string
     priority queue by the oracle's own tests); each parent's children are
-    2u (or 2 at depth n-2), BEGIN side first."""
+    2u (or 2 at depth n-2), BEGIN side first.  The large pools (budgeted)
+    take the chained multi-block select."""
     p = oracle.taillard(tid)[3]
     n = p.shape[1]
     o = oracle.Oracle(p)
-    _, st, _, tr = o.solve_batched(pool_size=pool, max_batches=nb, trace=2_000_000)
-    res, rounds = traced_solve(p, pool, nb)
-    tk = taken(tr)
+    _, st, _, tr = o.solve_batched(pool_size=pool, max_batches=nb, budget=budget, trace=2_000_000)
+    res, rounds = traced_solve(p, pool, nb, budget=budget)
+    # a budget stop leaves the last round's pops in the oracle's trace only
+    tk = {k: v[:st["decomposed"]] for k, v in taken(tr).items()}
     dev_perm = np.concatenate([r["permT"].T for r in rounds])
     dev_l1 = np.concatenate([r["l1"] for r in rounds])
     dev_l2 = np.concatenate([r["l2"] for r in rounds])

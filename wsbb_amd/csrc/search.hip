@@ -52,6 +52,8 @@ constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
 constexpr int kSelWin = 16384;  // candidates per select window after the first
 constexpr int kSelWin0 = 4096;  // candidates in the first window
 constexpr int kSettleK = 16;   // consecutive parents per settle thread
+constexpr int kSelChain = 8;   // blocks of the chained select (large pools)
+constexpr int kLink = 8;       // ints per hand-off: flag, stop, carry, nb, np, nc, pruned, consumed
 // largest cost a select key holds: bounds stay below 2^19, and a larger
 // host-given cost compares with every incumbent as this one does
 constexpr int kKeyCost = (1 << 23) - 1;
@@ -69,7 +71,8 @@ struct Ctrl {
   int push_total;
   int have_perm;
   int n_fb;  // children of the round listed for the full-LB2 fallback
-  int pad[7];
+  int sel_ticket;  // blocks of the chained select started this round
+  int pad[6];
   long long stats[WSBB_NSTATS];
   uint8_t best_perm[kMaxJobs];
 };
@@ -87,6 +90,7 @@ struct SArgs {
   int* s_cost;
   int* s_key;       // [CAP] what select reads of a node, one word:
                     // min(cost, kKeyCost) << 8 | depth
+  int* sel_link;    // [kSelChain][kLink] hand-off records of the chained select
   uint8_t* p_perm;  // [n][PCAP]
   int* p_l1;
   int* p_l2;
@@ -254,23 +258,24 @@ struct SelState {
   int carry, nb, np, nc, pruned, consumed, stop;
 };
 
+// The part of a window that needs no carried state: the words of the
+// candidates [consumed, consumed + win) and their prefix sums within the
+// window (in LDS, for the boundary search); word[] keeps each thread's own.
 template <int KPER>
-__device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, int* s_p,
-                                              int* s_w16, int* s_state, SelState& st) {
-  const int tid = threadIdx.x, lane = tid & 63, n = S.n;
-  const int win = min(KPER * 1024, top - st.consumed);
+__device__ __forceinline__ void window_words(const SArgs& S, int top, int inc, int consumed,
+                                             int win, int* s_p, int* s_w16, int (&word)[KPER]) {
+  const int tid = threadIdx.x, n = S.n;
   // coalesced loads: candidate i = t + 1024 k is the i-th from the top; its
   // reservation w (push_back1) and children c share one LDS word (both
-  // at most 2n)
-  // one key word per candidate (cost and depth); every load is issued
-  // before the first use (indices past the window are clamped to a valid
-  // slot and their words zeroed)
+  // at most 2n); one key word per candidate (cost and depth); every load is
+  // issued before the first use (indices past the window are clamped to a
+  // valid slot and their words zeroed)
   {
     int key[KPER];
 #pragma unroll
     for (int k = 0; k < KPER; ++k) {
       const int i = tid + 1024 * k;
-      key[k] = S.s_key[top - 1 - (st.consumed + min(i, win - 1))];
+      key[k] = S.s_key[top - 1 - (consumed + min(i, win - 1))];
     }
 #pragma unroll
     for (int k = 0; k < KPER; ++k) {
@@ -285,7 +290,7 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
   // blocked inclusive scan: thread t owns candidates [KPER t, KPER t + KPER)
   // and keeps their words in registers; the prefix sums go back to LDS for
   // the boundary search
-  int word[KPER], loc[KPER], tsum = 0;
+  int loc[KPER], tsum = 0;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
     word[k] = s_p[sel_pad(KPER * tid + k)];
@@ -297,6 +302,15 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
 #pragma unroll
   for (int k = 0; k < KPER; ++k) s_p[sel_pad(KPER * tid + k)] = off + loc[k];
   __syncthreads();
+}
+
+// The part that needs the carried state: sub-batch boundaries from the
+// carried reservation, then the taken parents and their first children.
+template <int KPER>
+__device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, int* s_p,
+                                              int* s_w16, int* s_state, SelState& st,
+                                              const int (&word)[KPER]) {
+  const int tid = threadIdx.x, lane = tid & 63;
   if (tid < 64) {
     int end = win, stop = 0, base = 0, cur = st.carry, nbw = st.nb;
     int i = wave_first_at_least(s_p, 0, win, S.pool_m1 - cur, lane);
@@ -354,6 +368,66 @@ __device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, 
   st.consumed += end;
 }
 
+template <int KPER>
+__device__ __forceinline__ void select_window(const SArgs& S, int top, int inc, int* s_p,
+                                              int* s_w16, int* s_state, SelState& st) {
+  const int win = min(KPER * 1024, top - st.consumed);
+  int word[KPER];
+  window_words<KPER>(S, top, inc, st.consumed, win, s_p, s_w16, word);
+  window_settle<KPER>(S, top, win, s_p, s_w16, s_state, st, word);
+}
+
+// the end of a select: overflow and budget checks, then the round's
+// parents, children and counters (one thread)
+__device__ void select_commit(const SArgs& S, Ctrl* C, const SelState& st, int top,
+                              long long decomposed0) {
+  const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
+  C->n_fb = 0;
+  // Overflow is decided here, before anything is committed: the round
+  // keeps at most nc children on top of the top - consumed nodes left
+  // below its parents, so if those fit, settle and push cannot run out of
+  // stack.  An overflowing round leaves the stack, incumbent and counters
+  // exactly as they were before it, and a search resumed from that state
+  // (with more capacity) visits the nodes an uninterrupted one would.
+  if (np > S.PCAP || nc > S.CCAP || static_cast<long long>(top - consumed) + nc > S.CAP) {
+    C->status = kOverflow;
+    C->n_parents = 0;
+    C->n_children = 0;
+    return;
+  }
+  if (S.budget > 0 && np > 0 && decomposed0 + np > S.budget) {
+    // stop before this round: the stack keeps every candidate, so a
+    // resumed search continues exactly where this one stopped
+    C->status = kBudget;
+    C->n_parents = 0;
+    C->n_children = 0;
+    return;
+  }
+  auto add = [&](int k, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[k]),
+              static_cast<unsigned long long>(v));
+  };
+  add(kStRounds, 1);
+  S.p_off[np] = nc;
+  if (pruned) add(kStPrunedPop, pruned);
+  C->stack_size = top - consumed;
+  if (np == 0) {
+    C->status = kDone;
+    C->n_parents = 0;
+    C->n_children = 0;
+  } else {
+    C->n_parents = np;
+    C->n_children = nc;
+    add(kStDecomposed, np);
+    add(kStBounded, nc);
+    if (S.count_pairs == 1) add(kStPairs, static_cast<long long>(nc) * S.P);
+    if (S.count_pairs == 2) {
+      add(kStPairs, static_cast<long long>(nc) * S.topk);
+      add(kStSubset, nc);
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_select(SArgs S) {
   __shared__ int s_p[kSelWin + kSelWin / 16];  // candidate words, then prefix sums
   __shared__ int s_w16[80];
@@ -395,53 +469,100 @@ __global__ __launch_bounds__(1024) void k_select(SArgs S) {
     select_window<kSelWin0 / 1024>(S, top, inc, s_p, s_w16, s_state, st);
   while (!st.stop && top - st.consumed > 0)
     select_window<kSelWin / 1024>(S, top, inc, s_p, s_w16, s_state, st);
-  const int np = st.np, nc = st.nc, pruned = st.pruned, consumed = st.consumed;
-  if (tid == 0) {
-    C->n_fb = 0;
-    // Overflow is decided here, before anything is committed: the round
-    // keeps at most nc children on top of the top - consumed nodes left
-    // below its parents, so if those fit, settle and push cannot run out of
-    // stack.  An overflowing round leaves the stack, incumbent and counters
-    // exactly as they were before it, and a search resumed from that state
-    // (with more capacity) visits the nodes an uninterrupted one would.
-    if (np > S.PCAP || nc > S.CCAP ||
-        static_cast<long long>(top - consumed) + nc > S.CAP) {
-      C->status = kOverflow;
-      C->n_parents = 0;
-      C->n_children = 0;
-      return;
-    }
-    if (S.budget > 0 && np > 0 && decomposed0 + np > S.budget) {
-      // stop before this round: the stack keeps every candidate, so a
-      // resumed search continues exactly where this one stopped
-      C->status = kBudget;
-      C->n_parents = 0;
-      C->n_children = 0;
-      return;
-    }
-    auto add = [&](int k, long long v) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[k]),
-                static_cast<unsigned long long>(v));
-    };
-    add(kStRounds, 1);
-    S.p_off[np] = nc;
-    if (pruned) add(kStPrunedPop, pruned);
-    C->stack_size = top - consumed;
-    if (np == 0) {
-      C->status = kDone;
-      C->n_parents = 0;
-      C->n_children = 0;
-    } else {
-      C->n_parents = np;
-      C->n_children = nc;
-      add(kStDecomposed, np);
-      add(kStBounded, nc);
-      if (S.count_pairs == 1) add(kStPairs, static_cast<long long>(nc) * S.P);
-      if (S.count_pairs == 2) {
-        add(kStPairs, static_cast<long long>(nc) * S.topk);
-        add(kStSubset, nc);
+  if (tid == 0) select_commit(S, C, st, top, decomposed0);
+}
+
+// ---- select for large pools: windows on kSelChain blocks ------------------
+// Block t (in the order blocks take tickets) turns window t (kSelWin
+// candidates) into words and prefix sums at once, then waits for block
+// t - 1's hand-off record (carried reservation, sub-batches, parents,
+// children, pruned, consumed, or a stop), settles its window from it and
+// hands on.  Every block that takes a ticket writes its hand-off on every
+// path (a stop when it has nothing to do), and block t waits only on
+// t - 1, which took its ticket earlier: the waits form a chain down to
+// ticket 0, which waits on nothing, so every wait ends.  The last block
+// continues alone if the round needs more windows; the block where the
+// round ends commits it (select_commit).
+__global__ __launch_bounds__(1024) void k_select_chain(SArgs S) {
+  constexpr int KPER = kSelWin / 1024;
+  __shared__ int s_p[kSelWin + kSelWin / 16];
+  __shared__ int s_w16[80];
+  __shared__ int s_state[4];
+  __shared__ int s_ticket;
+  __shared__ int s_link[kLink];
+  Ctrl* C = S.ctrl;
+  const int tid = threadIdx.x;
+  const int status = C->status, top = C->stack_size, inc = C->inc;
+  const long long decomposed0 = C->stats[kStDecomposed];
+  if (tid == 0) s_ticket = atomicAdd(&C->sel_ticket, 1);
+  __syncthreads();
+  const int t = s_ticket;
+  // tickets run 0..kSelChain-1 while expand clears the count after every
+  // select; a block beyond them (never the case) has no record to write
+  if (t >= kSelChain) return;
+  int* mine = S.sel_link + t * kLink;
+  auto hand_stop = [&]() {  // thread 0: this block hands on a stop
+    mine[1] = 1;
+    __hip_atomic_store(mine, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  if (status != kRunning || top == 0) {
+    // as k_select: nothing to pop
+    if (tid == 0) {
+      if (t == 0) {
+        if (status == kRunning) C->status = kDone;
+        C->n_parents = 0;
+        C->n_children = 0;
+        C->n_fb = 0;
       }
+      hand_stop();
     }
+    return;
+  }
+  const int w0 = t * kSelWin;
+  if (w0 >= top) {  // no candidates in this block's window
+    if (tid == 0) hand_stop();
+    return;
+  }
+  const int win = min(kSelWin, top - w0);
+  int word[KPER];
+  window_words<KPER>(S, top, inc, w0, win, s_p, s_w16, word);
+  SelState st{0, 0, 0, 0, 0, w0, 0};
+  if (t > 0) {
+    if (tid == 0) {
+      const int* prev = S.sel_link + (t - 1) * kLink;
+      while (__hip_atomic_load(prev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      for (int i = 1; i < kLink; ++i) s_link[i] = prev[i];
+    }
+    __syncthreads();
+    if (s_link[1]) {  // the round ended before this window: pass that on
+      if (tid == 0) hand_stop();
+      return;
+    }
+    // every earlier window was taken whole, so st.consumed == w0
+    st.carry = s_link[2];
+    st.nb = s_link[3];
+    st.np = s_link[4];
+    st.nc = s_link[5];
+    st.pruned = s_link[6];
+    st.consumed = s_link[7];
+  }
+  window_settle<KPER>(S, top, win, s_p, s_w16, s_state, st, word);
+  if (t == kSelChain - 1)
+    while (!st.stop && top - st.consumed > 0)
+      select_window<KPER>(S, top, inc, s_p, s_w16, s_state, st);
+  const bool done = st.stop || top - st.consumed <= 0;
+  if (tid == 0) {
+    mine[1] = done ? 1 : 0;
+    mine[2] = st.carry;
+    mine[3] = st.nb;
+    mine[4] = st.np;
+    mine[5] = st.nc;
+    mine[6] = st.pruned;
+    mine[7] = st.consumed;
+    __hip_atomic_store(mine, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (done) select_commit(S, C, st, top, decomposed0);
   }
 }
 
@@ -464,6 +585,12 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, i
   uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * T.n);    // [n][block] own columns
   const Ctrl* C = S.ctrl;
   const int np = C->n_parents;
+  // the chained select of the next round starts from cleared hand-offs
+  // (this kernel runs after every select and before the next)
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < kSelChain) S.sel_link[threadIdx.x * kLink] = 0;
+    if (threadIdx.x == 0) S.ctrl->sel_ticket = 0;
+  }
   // the grid covers the parent capacity: blocks past the round's parents
   // leave before staging
   if (static_cast<int>(blockIdx.x * blockDim.x) >= np) return;
@@ -1134,6 +1261,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_sl2 = take(4 * size_t(cap_nodes));
   const size_t o_scost = take(4 * size_t(cap_nodes));
   const size_t o_skey = take(4 * size_t(cap_nodes));
+  const size_t o_link = take(4 * size_t(kSelChain) * kLink);
   const size_t o_pperm = take(size_t(n) * PCAP);
   // parent arrays: PCAP + 1 entries plus room for the settle kernel's
   // 16-wide vector accesses past the last parent
@@ -1178,6 +1306,13 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.s_l2 = S->mem.as<int>(o_sl2);
   A.s_cost = S->mem.as<int>(o_scost);
   A.s_key = S->mem.as<int>(o_skey);
+  A.sel_link = S->mem.as<int>(o_link);
+  // the hand-off flags start cleared (afterwards expand clears them)
+  e = hipMemset(A.sel_link, 0, 4 * size_t(kSelChain) * kLink);
+  if (e != hipSuccess) {
+    destroy_search(ctx);
+    return hip_fail(e, "hipMemset(select hand-off)");
+  }
   A.p_perm = S->mem.as<uint8_t>(o_pperm);
   int** pints[13] = {&A.p_l1,      &A.p_l2,      &A.p_cost,    &A.p_sel,
                      &A.p_off,     &A.p_side,    &A.p_nch,     &A.p_leafmin, &A.p_leafmax,
@@ -1199,7 +1334,11 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   hipStream_t st = ctx->stream;
   const int rblocks = (A.PCAP + kRankWaves - 1) / kRankWaves;
   const int pblocks = (A.PCAP + 3) / 4;
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
+  // large pools: the chained select over several CUs
+  if (static_cast<long long>(A.pool_m1) * A.max_batches > 16LL * kSelWin0)
+    hipLaunchKernelGGL(k_select_chain, dim3(kSelChain), dim3(1024), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, A);
   // parent state for the int32 bound kernel (learned subsets, logging,
   // rounds below the two-node form's batch size); the two-node form of plain
   // LB2 runs its own recurrences
