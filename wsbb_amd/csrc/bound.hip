@@ -120,6 +120,38 @@ __device__ __forceinline__ void scan_pair32(const DevTables& T, int s, const uin
   }
 }
 
+// The same scan for a pair whose steps fit one batch (gpp <= CH), which also
+// issues scalar loads of the next pair's table lines (`s_next` >= 0) once
+// its own batch is in: the lines are in the scalar cache when the next
+// pair's batch is loaded, instead of each pair waiting a full miss.  At one
+// or two waves per SIMD (a small search round) that miss is not hidden by
+// other waves.  The loaded words are only made live after the next batch's
+// own wait (the scalar loads complete out of order, so that wait covers them
+// anyway) and never used otherwise.
+template <int NW>
+__device__ __forceinline__ void scan_pair32_next(const DevTables& T, int s, int s_next,
+                                                 const uint32_t (&um)[NW], int& D, int& U,
+                                                 int (&t)[4]) {
+  constexpr int GW = Rec32<NW>::GW, CH = Rec32<NW>::CH;
+  const int gpp = T.n4 >> 2;
+  Chunk32<NW, CH> c;
+  c.load(T.steps32 + static_cast<size_t>(s) * gpp * GW);
+  asm volatile("" ::"s"(t[0]), "s"(t[1]), "s"(t[2]), "s"(t[3]));
+  if (s_next >= 0) {
+    // every 64-byte line the next batch reads (it starts anywhere in a line
+    // and spans CH * GW dwords; the table has read-ahead padding after the
+    // last pair)
+    const int* nx = T.steps32 + static_cast<size_t>(s_next) * gpp * GW;
+    t[0] = ld_const(nx, 0);
+    t[1] = ld_const(nx, 16);
+    t[2] = ld_const(nx, 32);
+    t[3] = ld_const(nx, CH * GW - 1);
+  }
+#pragma unroll
+  for (int h = 0; h < CH; ++h)
+    if (h < gpp) group32<NW>(c.w + h * GW, um, D, U);
+}
+
 // MODE 0: plain LB2 (optionally with the `best` early exit) and nothing
 // else, so none of the optional outputs holds scalar registers in the pair
 // loop; MODE 1: LB2 with the optional outputs, pair masks and tile subsets;
@@ -376,14 +408,28 @@ __global__ __launch_bounds__(BLOCK) void bound_kernel(DevTables T, Src src, Boun
       if (s_lo < s_hi) {
         const int2 ab0 = ld_const2(T.pairs, s_lo);
         int a = ab0.x, b = ab0.y;
-        for (int s = s_lo; s < s_hi; ++s) {
-          int D = st32[a], U = st32[b];
-          const int one = one_of(a);
-          scan_pair32<NW>(T, s, um, D, U);
-          lb = max(lb, pair_value(one, b, U));
-          if (++b == m) {
-            ++a;
-            b = a + 1;
+        if (A.touch && (T.n4 >> 2) <= Rec32<NW>::CH) {
+          int t[4] = {0, 0, 0, 0};
+          for (int s = s_lo; s < s_hi; ++s) {
+            int D = st32[a], U = st32[b];
+            const int one = one_of(a);
+            scan_pair32_next<NW>(T, s, s + 1 < s_hi ? s + 1 : -1, um, D, U, t);
+            lb = max(lb, pair_value(one, b, U));
+            if (++b == m) {
+              ++a;
+              b = a + 1;
+            }
+          }
+        } else {
+          for (int s = s_lo; s < s_hi; ++s) {
+            int D = st32[a], U = st32[b];
+            const int one = one_of(a);
+            scan_pair32<NW>(T, s, um, D, U);
+            lb = max(lb, pair_value(one, b, U));
+            if (++b == m) {
+              ++a;
+              b = a + 1;
+            }
           }
         }
       }

@@ -74,6 +74,10 @@ struct BoundArgs {
   long long* stats;          // with tile_sel: [0] nodes, [1] full-LB2
                              // fallbacks, [2] node-pair evaluations
   int pk = 1;                // plain LB2 may run two nodes per lane (int16)
+  // plain LB2 over pairs whose steps fit one scalar batch loads the next
+  // pair's table lines ahead (set for launches of fewer waves than SIMDs,
+  // where nothing else hides a scalar-cache miss)
+  int touch = 0;
   // optional timestamps of the bound kernel's own dispatch (taken by the
   // launch itself, no separate marker packets in the stream)
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;

@@ -129,6 +129,15 @@ struct OpMin {
   __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
 
+// A kernel reads its control fields, then issues the loads it can make at
+// any index (clamped to the capacity) before it knows the round's parent
+// count, then calls this: loads written before it stay before it.  Without
+// it the compiler sinks a load whose value is only used after a branch on
+// the parent count below that branch, and the kernel makes two dependent
+// memory round trips instead of one.  No instruction and no wait: the
+// barrier only orders memory operations in the compiler.
+__device__ __forceinline__ void issue_before_branch() { asm volatile("" ::: "memory"); }
+
 // Wave-wide scans on DPP lane moves: row shifts by 1, 2, 4, 8 scan each
 // 16-lane row, the row broadcasts of lanes 15 and 31 carry the row totals
 // upward (GFX9 DPP controls 0x111-0x118, 0x142, 0x143; wave shift right by
@@ -584,23 +593,26 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(SArgs S, DevTables T, i
   int32_t* sp = lds;                                          // [n][MM] job-major
   uint8_t* pl = reinterpret_cast<uint8_t*>(lds + MM * T.n);    // [n][block] own columns
   const Ctrl* C = S.ctrl;
+  // the grid covers the parent capacity, so a thread has at most one
+  // parent: its stack slot and first child are loaded (at a clamped index,
+  // so in bounds whatever the round's count) together with the parent count
+  // and before the times are staged, and the memory round trips overlap
   const int np = C->n_parents;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pc = p < S.PCAP ? p : S.PCAP - 1;
+  const int si_any = S.p_sel[pc], off_any = S.p_off[pc];
+  issue_before_branch();
   // the chained select of the next round starts from cleared hand-offs
   // (this kernel runs after every select and before the next)
   if (blockIdx.x == 0) {
     if (threadIdx.x < kSelChain) S.sel_link[threadIdx.x * kLink] = 0;
     if (threadIdx.x == 0) S.ctrl->sel_ticket = 0;
   }
-  // the grid covers the parent capacity: blocks past the round's parents
-  // leave before staging
+  // blocks past the round's parents leave before staging
   if (static_cast<int>(blockIdx.x * blockDim.x) >= np) return;
-  // the grid covers the parent capacity, so a thread has at most one
-  // parent: its stack slot and first child load before the times are
-  // staged, and the two memory round trips overlap
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool mine_p = p < np;
-  const int si = mine_p ? S.p_sel[p] : 0;
-  const int off = mine_p ? S.p_off[p] : 0;
+  const int si = mine_p ? si_any : 0;
+  const int off = mine_p ? off_any : 0;
   stage_times<MM>(T, sp);
   __syncthreads();
   const int n = S.n, m = S.m, P = S.P;
@@ -1043,33 +1055,50 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
 __global__ __launch_bounds__(256) void k_push(SArgs S) {
   __shared__ int s_col[4][kMaxJobs];
   const Ctrl* C = S.ctrl;
-  // after an overflow or at the end of the search select left no parents
-  const int np = C->n_parents, base = C->push_base, total = C->push_total;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = S.n;
-  for (int pb = blockIdx.x * 4; pb < np; pb += gridDim.x * 4) {
+  const int stride = gridDim.x * 4;
+  // a parent's loads depend only on its index: the first pass's (at an
+  // index clamped to the capacity) go out with the control loads
+  bool first = true;
+  int kept = 0, koff = 0, side = 0, pl1 = 0, pl2 = 0;
+  // positions / ranks lane and lane + 64 (n <= 128)
+  int ord[2] = {0, 0}, cst[2] = {0, 0};
+  uint32_t col[2] = {0u, 0u};  // a register per byte: no wait before the control loads
+  auto load = [&](int p) {
+    kept = S.p_kept[p];
+    koff = S.p_koff[p];
+    side = S.p_side[p];
+    pl1 = S.p_l1[p];
+    pl2 = S.p_l2[p];
+    const int* ro = S.r_order + static_cast<size_t>(p) * n;
+    const int* rc = S.r_cost + static_cast<size_t>(p) * n;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int k = lane + 64 * t;
+      if (k < n) {
+        col[t] = S.p_perm[static_cast<size_t>(k) * S.PCAP + p];
+        ord[t] = ro[k];
+        cst[t] = rc[k];
+      }
+    }
+  };
+  // after an overflow or at the end of the search select left no parents
+  const int np = C->n_parents, base = C->push_base, total = C->push_total;
+  {
+    const int p0 = blockIdx.x * 4 + wave;
+    load(p0 < S.PCAP ? p0 : S.PCAP - 1);
+  }
+  issue_before_branch();
+  for (int pb = blockIdx.x * 4; pb < np; pb += stride) {
     const int p = pb + wave;
     const bool live = p < np;
-    int kept = 0, koff = 0, side = 0, pl1 = 0, pl2 = 0;
-    int ord[2] = {0, 0}, cst[2] = {0, 0};  // ranks lane and lane + 64 (n <= 128)
+    if (!first && live) load(p);
+    first = false;
     if (live) {
-      kept = S.p_kept[p];
-      koff = S.p_koff[p];
-      side = S.p_side[p];
-      pl1 = S.p_l1[p];
-      pl2 = S.p_l2[p];
-      for (int i = lane; i < n; i += 64)
-        s_col[wave][i] = S.p_perm[static_cast<size_t>(i) * S.PCAP + p];
-      const int* ro = S.r_order + static_cast<size_t>(p) * n;
-      const int* rc = S.r_cost + static_cast<size_t>(p) * n;
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int k = lane + 64 * t;
-        if (k < n) {
-          ord[t] = ro[k];
-          cst[t] = rc[k];
-        }
-      }
+      for (int t = 0; t < 2; ++t)
+        if (lane + 64 * t < n) s_col[wave][lane + 64 * t] = static_cast<int>(col[t]);
     }
     __syncthreads();
     if (live) {
@@ -1386,6 +1415,9 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
                : tiles < 2 * simds    ? 4
                : tiles < 4 * simds    ? 2
                                       : 1;
+    // table lines loaded a pair ahead: measured on 20x10 to pay at one
+    // 4096-child pool per round (256 waves) and to cost at ten (2560)
+    B.touch = static_cast<long long>(tiles) * B.groups < simds;
   }
   B.out = A.c_bound;
   B.feat_scale = static_cast<float>(ctx->inst.path_bound);
