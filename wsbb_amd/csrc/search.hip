@@ -52,6 +52,8 @@ constexpr int kSideBest = 99999;  // decompose_beginend.h:26, never updated
 constexpr int kSelWin = 16384;  // candidates per select window after the first
 constexpr int kSelWin0 = 4096;  // candidates in the first window
 constexpr int kSettleK = 16;   // consecutive parents per settle thread
+constexpr int kRankGrid = 4096;  // at most this many rank blocks (4 per settle thread)
+static_assert(kRankGrid == 4 * 1024, "each of the settle block's threads sums four rank blocks' counts");
 constexpr int kSelChain = 8;   // blocks of the chained select (large pools)
 constexpr int kLink = 8;       // ints per hand-off: flag, stop, carry, nb, np, nc, pruned, consumed
 // largest cost a select key holds: bounds stay below 2^19, and a larger
@@ -110,6 +112,7 @@ struct SArgs {
   int* c_bound;
   int* r_order;     // [PCAP][n] per parent, by rank: the chosen-side child index
   int* r_cost;      // [PCAP][n] per parent, by rank: that child's bound
+  int* r_leaves;    // [kRankGrid] leaf children counted by each rank block
   int* p_hr;        // [MM][PCAP + 1] heads of the parent's prefix
   int* p_hq;        // [MM][PCAP + 1] tails of the parent's suffix
   uint32_t* p_hm;   // [NW][PCAP + 1] the parent's scheduled jobs
@@ -802,7 +805,13 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
   if (tid == 0) *s_last = -1;
   int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
   int improved = 0, last = -1;
-  long long pruned = 0;
+  int pruned = 0;  // per thread: at most K parents' children per chunk, few chunks
+  // the rank blocks' leaf counts, four per thread
+  int leaves;
+  {
+    const int4 q = reinterpret_cast<const int4*>(S.r_leaves)[tid];
+    leaves = q.x + q.y + q.z + q.w;
+  }
   for (int c0 = 0; c0 < np; c0 += 1024 * K) {
     const int base = c0 + K * tid;  // arrays hold PCAP + 1 + K entries
     const bool any = base < np;
@@ -879,6 +888,7 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
   for (int d = 32; d >= 1; d >>= 1) {
     improved += __shfl_xor(improved, d, 64);
     pruned += __shfl_xor(pruned, d, 64);
+    leaves += __shfl_xor(leaves, d, 64);
   }
   if ((tid & 63) == 0) {
     if (improved)
@@ -887,6 +897,9 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
     if (pruned)
       atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStPrunedIns]),
                 static_cast<unsigned long long>(pruned));
+    if (leaves)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStLeaves]),
+                static_cast<unsigned long long>(leaves));
   }
   __syncthreads();
   const int bl = *s_last;
@@ -899,12 +912,13 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
     atomicMax(reinterpret_cast<unsigned long long*>(&C->stats[kStMaxPool]),
               static_cast<unsigned long long>(sbase + run_kept));
   }
-  // the leaf that set the incumbent: parent permutation with its swap
+  // the leaf that set the incumbent: parent permutation with its swap.  It
+  // is a child of leaf parent bl, whose children are numbered from p_off on
+  // the BEGIN side (child i places the job at position l1 + 1 + i, as expand
+  // wrote them), so the winner's fields come in one round of loads
   if (bl >= 0 && tid < n) {
-    const int bc = S.p_leafwin[bl];
-    const int d = S.c_desc[bc];
-    const int p = d >> 9, side = (d >> 8) & 1, j = d & 255;
-    const int sw0 = side == 0 ? S.p_l1[p] + 1 : S.p_l2[p] - 1;
+    const int bc = S.p_leafwin[bl], boff = S.p_off[bl], bl1 = S.p_l1[bl];
+    const int p = bl, sw0 = bl1 + 1, j = bl1 + 1 + (bc - boff);
     const int i = tid;
     const int src = (i == sw0) ? j : ((i == j) ? sw0 : i);
     C->best_perm[i] = S.p_perm[static_cast<size_t>(src) * S.PCAP + p];
@@ -941,7 +955,11 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = S.n;
   const int inc0 = C->inc;
-  long long leaves = 0;
+  // leaf children of the block's parents: one plain store per block (the
+  // settle kernel sums the blocks' counts); an atomic per wave into one
+  // counter serialises when a round is all leaf parents (ten thousand waves)
+  __shared__ int s_leaves;
+  if (threadIdx.x == 0) s_leaves = 0;
   for (int pb = blockIdx.x * kRankWaves; pb < np; pb += gridDim.x * kRankWaves) {
     const int p = pb + wave;
     const bool live = p < np;
@@ -1029,13 +1047,13 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
         S.p_leafmin[p] = simple ? lmin : INT_MAX;
         S.p_leafmax[p] = (simple && nch > 0) ? lmax : INT_MAX;
         S.p_leafwin[p] = simple ? lwin : -1;
-        if (simple) leaves += nch;
+        if (simple && nch > 0) atomicAdd(&s_leaves, nch);
       }
     }
     __syncthreads();
   }
-  if (lane == 0 && leaves) atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStLeaves]),
-                                     static_cast<unsigned long long>(leaves));
+  __syncthreads();
+  if (threadIdx.x == 0) S.r_leaves[blockIdx.x] = s_leaves;
 }
 
 __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
@@ -1301,6 +1319,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   // ranked rows: a parent has at most n chosen-side children
   const size_t o_rorder = take(4 * size_t(PCAP) * n);
   const size_t o_rcost = take(4 * size_t(PCAP) * n);
+  const size_t o_rleaves = take(4 * size_t(kRankGrid));
   // parent state rows: padded machine count, job-mask words
   const int m = ctx->T.m;
   const int mmp = m <= 8 ? 8 : (m <= 12 ? 12 : (m <= 16 ? 16 : (m <= 20 ? 20 : 24)));
@@ -1351,6 +1370,13 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.c_bound = S->mem.as<int>(o_cint[1]);
   A.r_order = S->mem.as<int>(o_rorder);
   A.r_cost = S->mem.as<int>(o_rcost);
+  A.r_leaves = S->mem.as<int>(o_rleaves);
+  // rank blocks past the grid never write their count
+  e = hipMemset(A.r_leaves, 0, 4 * size_t(kRankGrid));
+  if (e != hipSuccess) {
+    destroy_search(ctx);
+    return hip_fail(e, "hipMemset(rank counts)");
+  }
   A.p_hr = S->mem.as<int>(o_hr);
   A.p_hq = S->mem.as<int>(o_hq);
   A.p_hm = S->mem.as<uint32_t>(o_hm);
@@ -1499,7 +1525,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
       hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, L);
     }
   }
-  hipLaunchKernelGGL(k_rank, dim3(rblocks < 4096 ? rblocks : 4096), dim3(256), 0, st, A);
+  hipLaunchKernelGGL(k_rank, dim3(rblocks < kRankGrid ? rblocks : kRankGrid), dim3(256), 0, st, A);
   hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
   hipLaunchKernelGGL(k_push, dim3(pblocks < 4096 ? pblocks : 4096), dim3(256), 0, st, A);
   return hipGetLastError();
