@@ -131,6 +131,18 @@ def test_parents_follow_the_oracle_pop_sequence(tid, pool, nb, budget):
         assert np.array_equal(np.bincount(par, minlength=u.shape[0]), per)
         assert np.all(np.diff(par) >= 0)
     assert res["stats"]["bounded"] == sum(r["desc"].shape[0] for r in rounds)
+    # the work statistic (numerator of the search bound's roofline) sums,
+    # over exactly the decomposed parents, the algorithmic ops of their
+    # children: P(n + 4u + 4) + 2m d1 + m(m+1) d2 per child with prefix d1,
+    # suffix d2 and u free jobs
+    m = p.shape[0]
+    P, mm1 = m * (m - 1) // 2, m * (m + 1)
+    l1, l2 = tk["l1"].astype(np.int64), tk["l2"].astype(np.int64)
+    d1, d2, u = l1 + 1, n - l2, l2 - l1 - 1
+    base = P * (n + 4 * (u - 1) + 4)
+    w = np.where(d1 + d2 == n - 2, 2 * (P * (n + 4) + 2 * m * (d1 + 2) + mm1 * d2),
+                 u * (base + 2 * m * (d1 + 1) + mm1 * d2) + u * (base + 2 * m * d1 + mm1 * (d2 + 1)))
+    assert res["stats"]["work_ops"] == int(w.sum())
 
 
 def test_tied_leaves_keep_the_first_found_on_device():
