@@ -53,9 +53,11 @@ constexpr int kSelWin = 16384;  // candidates per select window after the first
 constexpr int kSelWin0 = 4096;  // candidates in the first window
 constexpr int kSettleK = 16;   // consecutive parents per settle thread
 constexpr int kRankGrid = 4096;  // at most this many rank blocks (4 per settle thread)
+constexpr int kSettleChunk = 1024 * kSettleK;  // parents per settle chunk
 static_assert(kRankGrid == 4 * 1024, "each of the settle block's threads sums four rank blocks' counts");
 constexpr int kSelChain = 8;   // blocks of the chained select (large pools)
 constexpr int kLink = 8;       // ints per hand-off: flag, stop, carry, nb, np, nc, pruned, consumed
+constexpr int kSettleChain = 8;  // blocks of the chained settle (large pools)
 // largest cost a select key holds: bounds stay below 2^19, and a larger
 // host-given cost compares with every incumbent as this one does
 constexpr int kKeyCost = (1 << 23) - 1;
@@ -74,7 +76,8 @@ struct Ctrl {
   int have_perm;
   int n_fb;  // children of the round listed for the full-LB2 fallback
   int sel_ticket;  // blocks of the chained select started this round
-  int pad[6];
+  int set_ticket;  // blocks of the chained settle started this round
+  int pad[5];
   long long stats[WSBB_NSTATS];
   uint8_t best_perm[kMaxJobs];
 };
@@ -93,6 +96,7 @@ struct SArgs {
   int* s_key;       // [CAP] what select reads of a node, one word:
                     // min(cost, kKeyCost) << 8 | depth
   int* sel_link;    // [kSelChain][kLink] hand-off records of the chained select
+  int* set_link;    // [kSettleChain][kLink] records of the chained settle
   uint8_t* p_perm;  // [n][PCAP]
   int* p_l1;
   int* p_l2;
@@ -795,24 +799,71 @@ __device__ __forceinline__ void st16(int* a, const int (&v)[kSettleK]) {
   for (int i = 0; i < kSettleK / 4; ++i) q[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
 }
 
-// np > 0 parents; s_w16 holds 40 ints, s_last one
-__device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
+// The chained settle's records: block t publishes, in S.set_link[t][f..f+2],
+// the flag and two values of phase f (0: its chunk's leaf minimum; 4: its
+// kept count and last improving parent).  chain_wait has wave 0 of block t
+// wait for phase f of every block before it (lane s for block s; all of them
+// took their tickets earlier and wait only on blocks before themselves, so
+// every wait ends) and combine their values: the minimum of the first value
+// (MIN) or their sum, with the maximum of the second, into s_pre[0..1].
+__device__ __forceinline__ void chain_put(const SArgs& S, int t, int f, int a, int b) {
+  int* rec = S.set_link + t * kLink;
+  rec[f + 1] = a;
+  rec[f + 2] = b;
+  __hip_atomic_store(rec + f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool MIN>
+__device__ __forceinline__ void chain_wait(const SArgs& S, int t, int f, int* s_pre) {
+  const int lane = threadIdx.x;
+  if (lane < 64) {
+    if (lane < t) {
+      const int* flag = S.set_link + lane * kLink + f;
+      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    int a = MIN ? INT_MAX : 0, b = -1;
+    if (lane < t) {
+      a = S.set_link[lane * kLink + f + 1];
+      b = S.set_link[lane * kLink + f + 2];
+    }
+    a = MIN ? wave_reduce(a, INT_MAX, OpMin()) : wave_reduce(a, 0, OpAdd());
+    b = wave_reduce(b, -1, OpMax());
+    if (lane == 0) {
+      s_pre[0] = a;
+      s_pre[1] = b;
+    }
+  }
+  __syncthreads();
+}
+
+// np > 0 parents; s_w16 holds 40 ints, s_last one, s_pre two.  One block
+// settles every chunk in order (CHAIN false), or block t of the chained
+// settle settles chunk t, taking the incumbent and the survivor offset its
+// chunk starts from out of the records of the blocks before it; the block
+// of the last chunk commits the round.
+template <bool CHAIN>
+__device__ void settle_round(const SArgs& S, int np, int t, int* s_w16, int* s_last, int* s_pre) {
   constexpr int K = kSettleK;
   Ctrl* C = S.ctrl;
   const int n = S.n, tid = threadIdx.x;
   const int inc0 = C->inc;
   const int sbase = C->stack_size;  // read with the other control fields
+  const bool final = !CHAIN || t == (np - 1) / kSettleChunk;
   if (tid == 0) *s_last = -1;
   int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
+  int pre_last = -1;                 // last improving parent of earlier blocks
   int improved = 0, last = -1;
   int pruned = 0;  // per thread: at most K parents' children per chunk, few chunks
-  // the rank blocks' leaf counts, four per thread
-  int leaves;
-  {
+  // the rank blocks' leaf counts, four per thread (added once per round)
+  int leaves = 0;
+  if (final) {
     const int4 q = reinterpret_cast<const int4*>(S.r_leaves)[tid];
     leaves = q.x + q.y + q.z + q.w;
   }
-  for (int c0 = 0; c0 < np; c0 += 1024 * K) {
+  const int c_first = CHAIN ? t * kSettleChunk : 0;
+  const int c_stop = CHAIN ? min(np, c_first + kSettleChunk) : np;
+  for (int c0 = c_first; c0 < c_stop; c0 += kSettleChunk) {
     const int base = c0 + K * tid;  // arrays hold PCAP + 1 + K entries
     const bool any = base < np;
     int lmin[K], kept[K], k0[K], nc[K];
@@ -834,7 +885,13 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
       tmin = min(tmin, lmin[k]);
     }
     int cmin;
-    int ib = min(run_min, block_exscan1024(tmin, INT_MAX, OpMin(), s_w16, &cmin));
+    const int xmin = block_exscan1024(tmin, INT_MAX, OpMin(), s_w16, &cmin);
+    if constexpr (CHAIN) {
+      if (tid == 0) chain_put(S, t, 0, cmin, 0);
+      chain_wait<true>(S, t, 0, s_pre);
+      run_min = min(inc0, s_pre[0]);
+    }
+    int ib = min(run_min, xmin);
     int tkept = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -869,8 +926,16 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
       }
       ib = min(ib, lmin[k]);
     }
+    if (last >= 0) atomicMax(s_last, last);  // final after the scan's barriers
     int csum;
-    int koff = run_kept + block_exscan1024(tkept, s_w16, &csum);
+    const int xsum = block_exscan1024(tkept, s_w16, &csum);
+    if constexpr (CHAIN) {
+      if (tid == 0) chain_put(S, t, 4, csum, *s_last);
+      chain_wait<false>(S, t, 4, s_pre);
+      run_kept = s_pre[0];
+      pre_last = s_pre[1];
+    }
+    int koff = run_kept + xsum;
     if (any) {
       st16(S.p_kept + base, kept);
 #pragma unroll
@@ -884,7 +949,6 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
     run_min = min(run_min, cmin);
     run_kept += csum;
   }
-  if (last >= 0) atomicMax(s_last, last);
   for (int d = 32; d >= 1; d >>= 1) {
     improved += __shfl_xor(improved, d, 64);
     pruned += __shfl_xor(pruned, d, 64);
@@ -901,8 +965,9 @@ __device__ void settle_round(const SArgs& S, int np, int* s_w16, int* s_last) {
       atomicAdd(reinterpret_cast<unsigned long long*>(&C->stats[kStLeaves]),
                 static_cast<unsigned long long>(leaves));
   }
+  if (!final) return;  // the block of the last chunk commits the round
   __syncthreads();
-  const int bl = *s_last;
+  const int bl = max(pre_last, *s_last);
   if (tid == 0) {
     if (bl >= 0) C->inc = run_min;
     C->push_base = sbase;
@@ -960,6 +1025,11 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
   // counter serialises when a round is all leaf parents (ten thousand waves)
   __shared__ int s_leaves;
   if (threadIdx.x == 0) s_leaves = 0;
+  // the chained settle that follows starts from cleared records
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < kSettleChain * kLink) S.set_link[threadIdx.x] = 0;
+    if (threadIdx.x == 0) C->set_ticket = 0;
+  }
   for (int pb = blockIdx.x * kRankWaves; pb < np; pb += gridDim.x * kRankWaves) {
     const int p = pb + wave;
     const bool live = p < np;
@@ -1058,10 +1128,31 @@ __global__ __launch_bounds__(256) void k_rank(SArgs S) {
 
 __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
   __shared__ int s_w16[40];
-  __shared__ int s_last;
+  __shared__ int s_last, s_pre[2];
   const int np = S.ctrl->n_parents;
   if (np == 0) return;
-  settle_round(S, np, s_w16, &s_last);
+  settle_round<false>(S, np, 0, s_w16, &s_last, s_pre);
+}
+
+// The chained settle for large pools: kSettleChain blocks, block t (in the
+// order blocks take tickets) settles parents [t, t + 1) kSettleChunk.  Its
+// chunk's leaf minimum and, once its incumbents are known, its kept count
+// are published at once and combined with those of the blocks before it
+// (chain_wait), so the chunks run side by side and only wait for two
+// records each.  A block whose chunk starts past the round's parents leaves
+// at once: only blocks after it would wait for it, and they have nothing
+// to settle either.
+__global__ __launch_bounds__(1024) void k_settle_chain(SArgs S) {
+  __shared__ int s_w16[40];
+  __shared__ int s_last, s_pre[2], s_ticket;
+  const int np = S.ctrl->n_parents;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&S.ctrl->set_ticket, 1);
+  __syncthreads();
+  const int t = s_ticket;
+  // tickets run 0..kSettleChain-1 (the rank kernel clears the count and the
+  // records before every settle)
+  if (t >= kSettleChain || np == 0 || t * kSettleChunk >= np) return;
+  settle_round<true>(S, np, t, s_w16, &s_last, s_pre);
 }
 
 // ---- push survivors: one wave per parent ------------------------------------
@@ -1309,6 +1400,7 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   const size_t o_scost = take(4 * size_t(cap_nodes));
   const size_t o_skey = take(4 * size_t(cap_nodes));
   const size_t o_link = take(4 * size_t(kSelChain) * kLink);
+  const size_t o_slink = take(4 * size_t(kSettleChain) * kLink);
   const size_t o_pperm = take(size_t(n) * PCAP);
   // parent arrays: PCAP + 1 entries plus room for the settle kernel's
   // 16-wide vector accesses past the last parent
@@ -1355,8 +1447,10 @@ static int ensure_search(wsbb_ctx* ctx, int pool_size, int max_batches, int cap_
   A.s_cost = S->mem.as<int>(o_scost);
   A.s_key = S->mem.as<int>(o_skey);
   A.sel_link = S->mem.as<int>(o_link);
-  // the hand-off flags start cleared (afterwards expand clears them)
+  A.set_link = S->mem.as<int>(o_slink);
+  // the hand-off flags start cleared (afterwards expand and rank clear them)
   e = hipMemset(A.sel_link, 0, 4 * size_t(kSelChain) * kLink);
+  if (e == hipSuccess) e = hipMemset(A.set_link, 0, 4 * size_t(kSettleChain) * kLink);
   if (e != hipSuccess) {
     destroy_search(ctx);
     return hip_fail(e, "hipMemset(select hand-off)");
@@ -1526,7 +1620,11 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
     }
   }
   hipLaunchKernelGGL(k_rank, dim3(rblocks < kRankGrid ? rblocks : kRankGrid), dim3(256), 0, st, A);
-  hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
+  // more parents than one settle chunk: the chunks on kSettleChain blocks
+  if (A.PCAP > kSettleChunk && A.PCAP <= kSettleChain * kSettleChunk)
+    hipLaunchKernelGGL(k_settle_chain, dim3(kSettleChain), dim3(1024), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
   hipLaunchKernelGGL(k_push, dim3(pblocks < 4096 ? pblocks : 4096), dim3(256), 0, st, A);
   return hipGetLastError();
 }
