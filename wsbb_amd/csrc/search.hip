@@ -53,11 +53,13 @@ constexpr int kSelWin = 16384;  // candidates per select window after the first
 constexpr int kSelWin0 = 4096;  // candidates in the first window
 constexpr int kSettleK = 16;   // consecutive parents per settle thread
 constexpr int kRankGrid = 4096;  // at most this many rank blocks (4 per settle thread)
-constexpr int kSettleChunk = 1024 * kSettleK;  // parents per settle chunk
+constexpr int kSettleChunk = 1024 * kSettleK;  // parents per chunk of the one-block settle
+constexpr int kChainK = 8;     // consecutive parents per thread of the chained settle
+constexpr int kChainChunk = 1024 * kChainK;    // parents per block of the chained settle
 static_assert(kRankGrid == 4 * 1024, "each of the settle block's threads sums four rank blocks' counts");
 constexpr int kSelChain = 8;   // blocks of the chained select (large pools)
 constexpr int kLink = 8;       // ints per hand-off: flag, stop, carry, nb, np, nc, pruned, consumed
-constexpr int kSettleChain = 8;  // blocks of the chained settle (large pools)
+constexpr int kSettleChain = 16;  // blocks of the chained settle (large pools)
 // largest cost a select key holds: bounds stay below 2^19, and a larger
 // host-given cost compares with every incumbent as this one does
 constexpr int kKeyCost = (1 << 23) - 1;
@@ -782,10 +784,11 @@ hipError_t launch_expand(const wsbb_ctx* ctx, const SArgs& A, int want_state, hi
 // starting incumbent, searched again (in the parent's ranked bounds) only for
 // parents after an improvement.  Prefix sums of the kept counts place the
 // survivors on the stack.
-__device__ __forceinline__ void ld16(const int* a, int (&v)[kSettleK]) {
+template <int K>
+__device__ __forceinline__ void ld16(const int* a, int (&v)[K]) {
   const int4* q = reinterpret_cast<const int4*>(a);
 #pragma unroll
-  for (int i = 0; i < kSettleK / 4; ++i) {
+  for (int i = 0; i < K / 4; ++i) {
     const int4 x = q[i];
     v[4 * i] = x.x;
     v[4 * i + 1] = x.y;
@@ -793,10 +796,11 @@ __device__ __forceinline__ void ld16(const int* a, int (&v)[kSettleK]) {
     v[4 * i + 3] = x.w;
   }
 }
-__device__ __forceinline__ void st16(int* a, const int (&v)[kSettleK]) {
+template <int K>
+__device__ __forceinline__ void st16(int* a, const int (&v)[K]) {
   int4* q = reinterpret_cast<int4*>(a);
 #pragma unroll
-  for (int i = 0; i < kSettleK / 4; ++i) q[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+  for (int i = 0; i < K / 4; ++i) q[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
 }
 
 // The chained settle's records: block t publishes, in S.set_link[t][f..f+2],
@@ -844,12 +848,13 @@ __device__ __forceinline__ void chain_wait(const SArgs& S, int t, int f, int* s_
 // of the last chunk commits the round.
 template <bool CHAIN>
 __device__ void settle_round(const SArgs& S, int np, int t, int* s_w16, int* s_last, int* s_pre) {
-  constexpr int K = kSettleK;
+  constexpr int K = CHAIN ? kChainK : kSettleK;  // parents per thread
+  constexpr int CHUNK = 1024 * K;
   Ctrl* C = S.ctrl;
   const int n = S.n, tid = threadIdx.x;
   const int inc0 = C->inc;
   const int sbase = C->stack_size;  // read with the other control fields
-  const bool final = !CHAIN || t == (np - 1) / kSettleChunk;
+  const bool final = !CHAIN || t == (np - 1) / CHUNK;
   if (tid == 0) *s_last = -1;
   int run_min = inc0, run_kept = 0;  // block-uniform carries over chunks
   int pre_last = -1;                 // last improving parent of earlier blocks
@@ -861,9 +866,9 @@ __device__ void settle_round(const SArgs& S, int np, int t, int* s_w16, int* s_l
     const int4 q = reinterpret_cast<const int4*>(S.r_leaves)[tid];
     leaves = q.x + q.y + q.z + q.w;
   }
-  const int c_first = CHAIN ? t * kSettleChunk : 0;
-  const int c_stop = CHAIN ? min(np, c_first + kSettleChunk) : np;
-  for (int c0 = c_first; c0 < c_stop; c0 += kSettleChunk) {
+  const int c_first = CHAIN ? t * CHUNK : 0;
+  const int c_stop = CHAIN ? min(np, c_first + CHUNK) : np;
+  for (int c0 = c_first; c0 < c_stop; c0 += CHUNK) {
     const int base = c0 + K * tid;  // arrays hold PCAP + 1 + K entries
     const bool any = base < np;
     int lmin[K], kept[K], k0[K], nc[K];
@@ -1135,7 +1140,7 @@ __global__ __launch_bounds__(1024) void k_settle(SArgs S) {
 }
 
 // The chained settle for large pools: kSettleChain blocks, block t (in the
-// order blocks take tickets) settles parents [t, t + 1) kSettleChunk.  Its
+// order blocks take tickets) settles parents [t, t + 1) kChainChunk.  Its
 // chunk's leaf minimum and, once its incumbents are known, its kept count
 // are published at once and combined with those of the blocks before it
 // (chain_wait), so the chunks run side by side and only wait for two
@@ -1151,7 +1156,7 @@ __global__ __launch_bounds__(1024) void k_settle_chain(SArgs S) {
   const int t = s_ticket;
   // tickets run 0..kSettleChain-1 (the rank kernel clears the count and the
   // records before every settle)
-  if (t >= kSettleChain || np == 0 || t * kSettleChunk >= np) return;
+  if (t >= kSettleChain || np == 0 || t * kChainChunk >= np) return;
   settle_round<true>(S, np, t, s_w16, &s_last, s_pre);
 }
 
@@ -1621,7 +1626,7 @@ static hipError_t launch_round(wsbb_ctx* ctx, const SArgs& A, int kind, hipEvent
   }
   hipLaunchKernelGGL(k_rank, dim3(rblocks < kRankGrid ? rblocks : kRankGrid), dim3(256), 0, st, A);
   // more parents than one settle chunk: the chunks on kSettleChain blocks
-  if (A.PCAP > kSettleChunk && A.PCAP <= kSettleChain * kSettleChunk)
+  if (A.PCAP > kSettleChunk && A.PCAP <= kSettleChain * kChainChunk)
     hipLaunchKernelGGL(k_settle_chain, dim3(kSettleChain), dim3(1024), 0, st, A);
   else
     hipLaunchKernelGGL(k_settle, dim3(1), dim3(1024), 0, st, A);
