@@ -88,8 +88,9 @@ def test_budget_stops_without_proof():
 def test_graph_replay_matches_direct_launches(rounds):
     """Rounds replayed from a captured hipGraph (many replays per solve, host
     checks between them) give the same search as direct launches, which the
-    tests above compare with the oracle."""
-    for tid, pool, nb in ((1, 4096, 1), (11, 4096, 10), (3, 300, 3)):
+    tests above compare with the oracle.  ta021 at 65536 x 8 takes the
+    chained select and the chained settle."""
+    for tid, pool, nb in ((1, 4096, 1), (11, 4096, 10), (3, 300, 3), (21, 65536, 8)):
         p = oracle.taillard(tid)[3]
         with wsbb_amd.Engine(p) as eng:
             eng.search_options(use_graph=0, rounds_per_sync=rounds)
