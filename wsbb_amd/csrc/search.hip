@@ -322,12 +322,13 @@ __device__ __forceinline__ void window_words(const SArgs& S, int top, int inc, i
   __syncthreads();
 }
 
-// The part that needs the carried state: sub-batch boundaries from the
-// carried reservation, then the taken parents and their first children.
+// The part that needs the carried state.  window_bounds: the sub-batch
+// boundaries from the carried reservation (wave 0); returns the end of the
+// round's candidates in the window (win unless the last sub-batch fills in
+// it) and carries the reservation, sub-batch count and stop on.
 template <int KPER>
-__device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, int* s_p,
-                                              int* s_w16, int* s_state, SelState& st,
-                                              const int (&word)[KPER]) {
+__device__ __forceinline__ int window_bounds(const SArgs& S, int win, int* s_p, int* s_state,
+                                             SelState& st) {
   const int tid = threadIdx.x, lane = tid & 63;
   if (tid < 64) {
     int end = win, stop = 0, base = 0, cur = st.carry, nbw = st.nb;
@@ -356,7 +357,19 @@ __device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, 
   st.stop = s_state[1];
   st.carry = s_state[2];
   st.nb = s_state[3];
-  // parents taken in [0, end): candidates whose prefix steps up (w > 0)
+  return end;
+}
+
+// window_take: the parents taken in [0, end) (candidates whose prefix steps
+// up, w > 0) get their stack slots and first children from the running
+// counts; the taken ones' slot and child offsets within the window are a
+// block scan, or come precomputed (pre, for end == win)
+struct TakeScan {
+  int slot, coff, ntaken, nchild;
+};
+template <int KPER>
+__device__ __forceinline__ TakeScan window_scan(int end, int* s_w16, const int (&word)[KPER]) {
+  const int tid = threadIdx.x;
   int mine = 0, mc = 0;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
@@ -364,10 +377,15 @@ __device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, 
     mine += t;
     mc += t ? (word[k] >> 16) : 0;
   }
-  int slot, coff, ntaken, nchild;
-  block_exscan1024_2(mine, mc, s_w16, &slot, &coff, &ntaken, &nchild);
-  slot += st.np;
-  coff += st.nc;
+  TakeScan r;
+  block_exscan1024_2(mine, mc, s_w16, &r.slot, &r.coff, &r.ntaken, &r.nchild);
+  return r;
+}
+template <int KPER>
+__device__ __forceinline__ void window_write(const SArgs& S, int top, int end, const TakeScan& r,
+                                             const SelState& st, const int (&word)[KPER]) {
+  const int tid = threadIdx.x;
+  int slot = r.slot + st.np, coff = r.coff + st.nc;
 #pragma unroll
   for (int k = 0; k < KPER; ++k) {
     const int i = KPER * tid + k;
@@ -380,10 +398,22 @@ __device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, 
       coff += word[k] >> 16;
     }
   }
-  st.np += ntaken;
-  st.nc += nchild;
-  st.pruned += end - ntaken;
+}
+__device__ __forceinline__ void window_count(SelState& st, int end, const TakeScan& r) {
+  st.np += r.ntaken;
+  st.nc += r.nchild;
+  st.pruned += end - r.ntaken;
   st.consumed += end;
+}
+
+template <int KPER>
+__device__ __forceinline__ void window_settle(const SArgs& S, int top, int win, int* s_p,
+                                              int* s_w16, int* s_state, SelState& st,
+                                              const int (&word)[KPER]) {
+  const int end = window_bounds<KPER>(S, win, s_p, s_state, st);
+  const TakeScan r = window_scan<KPER>(end, s_w16, word);
+  window_write<KPER>(S, top, end, r, st, word);
+  window_count(st, end, r);
 }
 
 template <int KPER>
@@ -544,6 +574,10 @@ __global__ __launch_bounds__(1024) void k_select_chain(SArgs S) {
   const int win = min(kSelWin, top - w0);
   int word[KPER];
   window_words<KPER>(S, top, inc, w0, win, s_p, s_w16, word);
+  // the window's parents and children as if the round does not stop in it
+  // (it stops only in the window where its last sub-batch fills): scanned
+  // before the wait, off the chain of hand-offs
+  const TakeScan whole = window_scan<KPER>(win, s_w16, word);
   SelState st{0, 0, 0, 0, 0, w0, 0};
   if (t > 0) {
     if (tid == 0) {
@@ -566,12 +600,7 @@ __global__ __launch_bounds__(1024) void k_select_chain(SArgs S) {
     st.pruned = s_link[6];
     st.consumed = s_link[7];
   }
-  window_settle<KPER>(S, top, win, s_p, s_w16, s_state, st, word);
-  if (t == kSelChain - 1)
-    while (!st.stop && top - st.consumed > 0)
-      select_window<KPER>(S, top, inc, s_p, s_w16, s_state, st);
-  const bool done = st.stop || top - st.consumed <= 0;
-  if (tid == 0) {
+  auto hand_on = [&](bool done) {  // thread 0
     mine[1] = done ? 1 : 0;
     mine[2] = st.carry;
     mine[3] = st.nb;
@@ -580,6 +609,30 @@ __global__ __launch_bounds__(1024) void k_select_chain(SArgs S) {
     mine[6] = st.pruned;
     mine[7] = st.consumed;
     __hip_atomic_store(mine, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  const int end = window_bounds<KPER>(S, win, s_p, s_state, st);
+  bool handed = false;
+  if (end == win) {
+    // the whole window is the round's: the next block gets its record
+    // before this one writes its parents (into slots of its own)
+    const SelState at = st;
+    window_count(st, win, whole);
+    if (t != kSelChain - 1) {
+      if (tid == 0) hand_on(top - st.consumed <= 0);
+      handed = true;
+    }
+    window_write<KPER>(S, top, win, whole, at, word);
+  } else {
+    const TakeScan r = window_scan<KPER>(end, s_w16, word);
+    window_write<KPER>(S, top, end, r, st, word);
+    window_count(st, end, r);
+  }
+  if (t == kSelChain - 1)
+    while (!st.stop && top - st.consumed > 0)
+      select_window<KPER>(S, top, inc, s_p, s_w16, s_state, st);
+  const bool done = st.stop || top - st.consumed <= 0;
+  if (tid == 0) {
+    if (!handed) hand_on(done);
     if (done) select_commit(S, C, st, top, decomposed0);
   }
 }
